@@ -1,0 +1,146 @@
+/*
+ * psv_oracle.h -- CPU restatement of the reference 2D elastic P-SV FWI hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests,
+ * the smoke check and the CPU-baseline leg of the benchmark link it.  It is
+ * never part of the product path (the product is the HIP library under
+ * cuda_fwi_amd/csrc, which fails loudly when it is not built).
+ *
+ * Every function restates one reference function statement by statement,
+ * in the same floating-point operation order, so that an FP64 run reproduces
+ * the reference CPU path (real_sim = double, CUDA_week2/globvar.cuh:16).
+ * The reference only implements fdorder == 2; the fdorder == 4 branch is an
+ * extension (Holberg hc[1], hc[2]) that reduces to the reference at order 2.
+ *
+ * Layout: every 2D array is flat row-major.  Full-grid arrays are
+ * [nzt][nxt]; the averaged material arrays mu_zx, rho_zp, rho_xp are
+ * [nzt-1][nxt-1] (CUDA_week2/fd_medium.cu:128-159); CPML x-type memory
+ * arrays are [nzt][2(npml+1)], z-type ones [2(npml+1)][nxt]
+ * (CUDA_week2/fd_cpml.cu:211-229); the forward store is [nft][nfz][nfx]
+ * per field and the gradients are [nfz][nfx] (CUDA_week2/fd_fwi.cu:65-78).
+ *
+ * Pinning status (each function is checked by tests/test_oracle_*.py):
+ *   psvo_holbergcoeff        table equality + Taylor row closed form
+ *   psvo_cpml_pro            closed-form profile, invariants
+ *   psvo_velocity_to_lame    closed form
+ *   psvo_average_parameters  closed form (harmonic / inverse arithmetic)
+ *   psvo_create_wavelet      reference src_signal_*.csv (6 sig. digits)
+ *   psvo_forward_*           reference rec_signal.csv (3 receivers x 334
+ *                            steps, incl. 1e-149 light-cone precursors),
+ *                            linearity, reciprocity, energy decay
+ *   psvo_adjoint_*           time-reversal identity with psvo_forward_*,
+ *                            gradient direction against central finite
+ *                            differences of the misfit (sign/term checks)
+ *   psvo_l2_adjoint_sources  closed form
+ *   psvo_calc_opt_step       numpy.polyfit vertex
+ *   psvo_scale_gradients_energy, psvo_fwi_grad_update   parity unpinned
+ *                            beyond closed-form checks on tiny inputs
+ */
+#ifndef PSV_ORACLE_H
+#define PSV_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid / scheme / FWI-window description shared by forward and adjoint.
+ * Mirrors the scalar arguments of forward_kernel_PSV
+ * (CUDA_week2/fd_kernel.cuh:12-40). */
+typedef struct psvo_cfg {
+    int nt, nzt, nxt, fpad, ppad;
+    double dt, dx, dz;
+    int fsurf;          /* 1 = free surface on the top row iz == fpad */
+    int fdorder;        /* 2 (reference) or 4 (extension) */
+    double hc[7];       /* Holberg coefficients, hc[1..fdorder/2] used */
+    int npml;
+    int fwinv;          /* 1 = store forward fields / correlate gradients */
+    int fwi_dt, fwi_dx, fwi_dz;
+    int fwi_x1, fwi_x2, fwi_z1, fwi_z2;
+} psvo_cfg;
+
+/* Source or receiver geometry (padded grid indices). comp: 1 = vx,
+ * 2 = vz (the only type in the reference), 4 = explosive (sxx and szz,
+ * sources only; an extension). */
+typedef struct psvo_seis {
+    int n;
+    const int* x;
+    const int* z;
+    const int* comp;
+    const int* shot;
+} psvo_seis;
+
+/* ---- setup functions (FP64) ---- */
+void psvo_holbergcoeff(int fdorder, int maxrelerr, double hc[7]);
+void psvo_cpml_pro(int npml, double dh, double fpml, double npower, double k_max_pml,
+                   double rcoef, double damp_v_pml, double dt,
+                   double* a, double* b, double* K,
+                   double* a_half, double* b_half, double* K_half);
+void psvo_velocity_to_lame(double vp, double vs, double rho, double* lam, double* mu);
+void psvo_average_parameters(int dimz, int dimx, const double* mu, const double* rho,
+                             double* mu_zx, double* rho_zp, double* rho_xp);
+void psvo_stagger_over_grid_modify(int dimz, int dimx, double scalar_rho,
+                                   double scalar_lam, double scalar_mu,
+                                   double* rho, double* lam, double* mu);
+void psvo_create_wavelet(double* signal, int nt, double dt, double amp, double fc,
+                         double ts, int shape);
+
+/* ---- misfit, energy scaling, model update, step length (FP64) ---- */
+/* returns 0, or 1 when |residual| > 1e10 (the reference aborts there) */
+int psvo_l2_adjoint_sources(int nr, int nt, double dt, double* signal,
+                            const double* signal_meas, double* misfit);
+void psvo_scale_gradients_energy(const double* Ws, const double* Wr, int nxt,
+                                 double C_rho, double C_lam, double C_mu,
+                                 int nfz, int nfx, int z1, int x1, int fdz, int fdx,
+                                 double* grad_lam, double* grad_mu, double* grad_rho);
+void psvo_medium_average(const double* lam, const double* mu, const double* rho,
+                         int dimz, int dimx, double* C_lam, double* C_mu, double* C_rho);
+void psvo_fwi_grad_update(double* mat, double* mat_new, const double* grad, int nxt,
+                          double step_length, int dimz, int dimx, int z1, int x1,
+                          int fwi_dz, int fwi_dx, int override_old, int mat_type,
+                          int nfx);
+double psvo_calc_opt_step(const double L2[3], const double sl[3]);
+
+/* ---- propagators (FP64 "_d" and FP32 "_f" builds of one template) ---- */
+/* pml: 6 consecutive vectors of length 2(npml+1): a, b, K, a_half, b_half, K_half.
+ * mem: 8 consecutive arrays: mem_vx_x, mem_vz_x, mem_sxx_x, mem_szx_x
+ *      (each [nzt][2(npml+1)]) then mem_vx_z, mem_vz_z, mem_szz_z, mem_szx_z
+ *      (each [2(npml+1)][nxt]).
+ * store: 5 consecutive arrays [nft][nfz][nfx]: vx/dt, vz/dt, sxx, szx, szz.
+ * Returns 0 on success, negative on invalid arguments. */
+int psvo_forward_d(const psvo_cfg* c, int ishot,
+                   double* vx, double* vz, double* sxx, double* szx, double* szz, double* We,
+                   const double* lam, const double* mu, const double* mu_zx,
+                   const double* rho_zp, const double* rho_xp,
+                   const double* pml, double* mem,
+                   const psvo_seis* src, const double* src_signal,
+                   const psvo_seis* rec, double* rec_signal,
+                   double* store);
+int psvo_adjoint_d(const psvo_cfg* c, int ishot,
+                   double* vx, double* vz, double* sxx, double* szx, double* szz, double* We,
+                   const double* lam, const double* mu, const double* mu_zx,
+                   const double* rho_zp, const double* rho_xp,
+                   const double* pml, double* mem,
+                   const psvo_seis* src, const double* src_signal,
+                   const double* store,
+                   double* grad_lam, double* grad_mu, double* grad_rho);
+int psvo_forward_f(const psvo_cfg* c, int ishot,
+                   float* vx, float* vz, float* sxx, float* szx, float* szz, float* We,
+                   const float* lam, const float* mu, const float* mu_zx,
+                   const float* rho_zp, const float* rho_xp,
+                   const float* pml, float* mem,
+                   const psvo_seis* src, const float* src_signal,
+                   const psvo_seis* rec, float* rec_signal,
+                   float* store);
+int psvo_adjoint_f(const psvo_cfg* c, int ishot,
+                   float* vx, float* vz, float* sxx, float* szx, float* szz, float* We,
+                   const float* lam, const float* mu, const float* mu_zx,
+                   const float* rho_zp, const float* rho_xp,
+                   const float* pml, float* mem,
+                   const psvo_seis* src, const float* src_signal,
+                   const float* store,
+                   float* grad_lam, float* grad_mu, float* grad_rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

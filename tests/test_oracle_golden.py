@@ -1,0 +1,254 @@
+"""Pins the CPU oracle against the reference's own output files and against
+closed forms, library routines and invariants.
+
+Golden data (tests/golden/ref_forward_run.npz) are the reference's shipped
+outputs of one forward run, written with 6 significant digits, so the
+comparison tolerance is half a unit in the 6th digit (5e-6 relative) and
+every exact zero must be reproduced (the numerical light cone).
+"""
+import numpy as np
+import pytest
+
+import psv_oracle as O
+import refcase as RC
+
+SIX_DIGITS = 5.0e-6
+
+
+def _match_6_digits(ours, ref):
+    ref = np.asarray(ref)
+    assert np.array_equal(ours == 0, ref == 0), "zero pattern (light cone) differs"
+    nz = ref != 0
+    rel = np.abs(ours[nz] - ref[nz]) / np.abs(ref[nz])
+    return rel.max() if rel.size else 0.0
+
+
+def test_rec_signal_matches_reference_output(golden):
+    """rec_signal.csv: 3 receivers x 334 steps incl. 1e-149 precursors."""
+    out = O.forward(RC.reference_run())
+    assert _match_6_digits(out["rec"], golden["rec_signal"]) <= SIX_DIGITS
+
+
+@pytest.mark.parametrize("k", [39, 52, 66])
+def test_vz_snapshots_match_reference_output(golden, k):
+    """vz_snap<k>.csv = vz after 5k+1 updates; snapshot 66 reaches the x CPML strips."""
+    out = O.forward(RC.reference_run(nt=5 * k + 1))
+    rows, cols = golden["snap_rows"], golden["snap_cols"]
+    assert _match_6_digits(out["vz"][np.ix_(rows, cols)], golden[f"snap{k}"]) <= SIX_DIGITS
+    strips = out["vz"][:, np.r_[0:12, 611:623]]
+    assert _match_6_digits(strips, golden[f"snap{k}_strips"]) <= SIX_DIGITS
+    if k == 66:
+        assert np.count_nonzero(golden["snap66_strips"]) > 500
+
+
+def test_wavelet_matches_reference_output(golden):
+    s = O.create_wavelet(2500, 0.2e-3, 1000.0, 15.0, 0.0, 1)
+    assert _match_6_digits(s, golden["src_signal_row"]) <= SIX_DIGITS
+
+
+def test_wavelet_closed_form():
+    nt, dt, fc, ts, amp = 400, 1e-3, 12.0, 0.01, 3.0
+    t = np.arange(nt) * dt
+    tau = 3.14159265 * (t - 1.5 / fc - ts) / (1.5 / fc)
+    ref = amp * (1 - 2 * tau ** 2) * np.exp(-2 * tau ** 2)
+    assert np.allclose(O.create_wavelet(nt, dt, amp, fc, ts), ref, rtol=1e-13, atol=1e-15)
+    # peak at t0 = 1.5/fc + ts
+    assert abs(np.argmax(O.create_wavelet(nt, dt, amp, fc, ts)) * dt - (1.5 / fc + ts)) <= dt
+
+
+def test_holberg_taylor_rows_are_textbook_weights():
+    assert np.allclose(O.holbergcoeff(2, 0)[1:2], [1.0])
+    assert np.allclose(O.holbergcoeff(4, 0)[1:3], [9 / 8, -1 / 24])
+    assert np.allclose(O.holbergcoeff(6, 0)[1:4], [75 / 64, -25 / 384, 3 / 640])
+    # staggered first-derivative consistency: sum (2k-1) c_k = 1 for Taylor
+    for order in (2, 4, 6, 8, 10, 12):
+        hc = O.holbergcoeff(order, 0)
+        k = np.arange(1, order // 2 + 1)
+        assert abs(np.sum((2 * k - 1) * hc[1:order // 2 + 1]) - 1) < 1e-12
+    # Holberg rows deviate from exact consistency by their error class only
+    hc = O.holbergcoeff(4, 1)
+    assert np.allclose(hc[1:3], [1.1382, -0.046414])
+    assert abs(hc[1] + 3 * hc[2] - 1) < 1e-2
+
+
+def test_cpml_profile_invariants():
+    npml, dh, fpml, dt = 12, 5.0, 15.0, 5e-4
+    p = O.cpml_pro(npml, dh, fpml, 2.0, 1.0, 1e-3, 2750.0, dt)
+    a, b, K, ah, bh, Kh = p
+    assert np.all((b > 0) & (b <= 1)) and np.all((bh > 0) & (bh <= 1))
+    assert np.all(a <= 0) and np.all(ah <= 0)
+    assert np.all(K == 1) and np.all(Kh == 1)        # k_max_PML = 1
+    # damping is strongest at the outer edge and vanishes at the inner edge
+    assert a[0] < a[npml // 2] < 0 and a[npml] == 0
+    assert a[2 * npml + 1] < a[npml + npml // 2] < 0 and a[npml + 1] == 0
+    # the right side mirrors the left: a[npml+1+k] == a[npml-k]
+    assert np.allclose(p[0, npml + 1:], p[0, npml::-1], rtol=1e-12)
+    # closed form at the outer edge: d = d0, alpha = 0, b = exp(-d0 dt), a = b - 1
+    d0 = -3 * 2750.0 * np.log(1e-3) / (2 * npml * dh)
+    assert np.isclose(b[0], np.exp(-d0 * dt), rtol=1e-14)
+    assert np.isclose(a[0], b[0] - 1, rtol=1e-12)
+    # K_max > 1 grows K towards the edge
+    p2 = O.cpml_pro(npml, dh, fpml, 2.0, 3.0, 1e-3, 2750.0, dt)
+    assert np.isclose(p2[2, 0], 3.0) and p2[2, npml] == 1.0
+
+
+def test_lame_and_averaging_closed_forms():
+    lam, mu = O.velocity_to_lame(2500.0, 1443.0, 2000.0)
+    assert np.isclose(mu, 1443.0 ** 2 * 2000.0) and np.isclose(lam, 2500.0 ** 2 * 2000 - 2 * mu)
+    from scipy.stats import hmean
+    rng = np.random.default_rng(1)
+    mu = rng.uniform(1e9, 5e9, (7, 9))
+    rho = rng.uniform(1500, 2500, (7, 9))
+    mu_zx, rho_zp, rho_xp = O.average_parameters(mu, rho)
+    for iz in range(6):
+        for ix in range(8):
+            q = [mu[iz, ix], mu[iz, ix + 1], mu[iz + 1, ix], mu[iz + 1, ix + 1]]
+            assert np.isclose(mu_zx[iz, ix], hmean(q), rtol=1e-13)
+            assert np.isclose(rho_zp[iz, ix], 1 / np.mean([rho[iz, ix], rho[iz + 1, ix]]), rtol=1e-14)
+            assert np.isclose(rho_xp[iz, ix], 1 / np.mean([rho[iz, ix], rho[iz, ix + 1]]), rtol=1e-14)
+    mu[3, 4] = 0.0
+    mu_zx, _, _ = O.average_parameters(mu, rho)
+    assert mu_zx[2, 3] == 0 and mu_zx[3, 4] == 0 and mu_zx[2, 4] == 0 and mu_zx[3, 3] == 0
+
+
+def test_misfit_closed_form():
+    rng = np.random.default_rng(2)
+    syn, obs = rng.standard_normal((4, 50)), rng.standard_normal((4, 50))
+    m, res, st = O.l2_adjoint_sources(1e-3, syn, obs)
+    assert st == 0 and np.array_equal(res, syn - obs)
+    assert np.isclose(m, 0.5 * 1e-3 * np.sum((syn - obs) ** 2), rtol=1e-13)
+    syn[2, 7] = 1e11
+    assert O.l2_adjoint_sources(1e-3, syn, obs)[2] == 1
+
+
+def test_parabola_vertex_matches_polyfit():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        sl = np.sort(rng.uniform(0, 1, 3))
+        c = rng.uniform(0.5, 2.0), rng.uniform(-2, 2), rng.uniform(0.1, 3)
+        L2 = c[0] * sl ** 2 + c[1] * sl + c[2] + rng.normal(0, 0.05, 3)
+        p = np.polyfit(sl, L2, 2)
+        assert np.isclose(O.calc_opt_step(L2, sl), -p[1] / (2 * p[0]), rtol=1e-8)
+
+
+def test_forward_is_linear_in_source_amplitude():
+    """Doubling every source doubles every output bit for bit (FP64)."""
+    pb = RC.small_problem(nt=150)
+    a = O.forward(pb)
+    pb.src_signal = pb.src_signal * 2.0
+    b = O.forward(pb)
+    for k in ("rec", "vx", "vz", "sxx", "szx", "szz"):
+        assert np.array_equal(b[k], 2.0 * a[k])
+
+
+def test_forward_reciprocity_homogeneous():
+    """vz force at A recorded at B equals vz force at B recorded at A."""
+    pb = RC.small_problem(nx=50, nz=40, nt=250, fsurf=0, hetero=False, nsrc=1, nrec=1)
+    A, B = (pb.ppad + 12, pb.ppad + 10), (pb.ppad + 25, pb.ppad + 37)
+    pb.src = dict(x=[A[1]], z=[A[0]], comp=[2], shot=[0])
+    pb.rec = dict(x=[B[1]], z=[B[0]], comp=[2], shot=[0])
+    ab = O.forward(pb)["rec"][0]
+    pb.src, pb.rec = dict(pb.rec), dict(pb.src)
+    ba = O.forward(pb)["rec"][0]
+    # exact in the interior stencil; the CPML strips break it at the 1e-7 level
+    assert np.linalg.norm(ab - ba) / np.linalg.norm(ab) < 1e-5
+
+
+def test_cpml_absorbs_energy():
+    """With damping the late-time energy is far below the undamped run."""
+    pb = RC.small_problem(nx=40, nz=40, nt=900, fsurf=0, hetero=False, nsrc=1, nrec=1)
+    e_damp = sum(np.sum(O.forward(pb)[k] ** 2) for k in ("vx", "vz"))
+    pb.pml = np.vstack([np.zeros(pb.pml.shape[1]), np.ones(pb.pml.shape[1]),
+                        np.ones(pb.pml.shape[1])] * 2)
+    e_free = sum(np.sum(O.forward(pb)[k] ** 2) for k in ("vx", "vz"))
+    assert e_damp < 1e-3 * e_free
+
+
+@pytest.mark.parametrize("fsurf", [0, 1])
+@pytest.mark.parametrize("fdorder", [2, 4])
+def test_adjoint_propagation_is_time_reversed_forward(fsurf, fdorder):
+    """adjoint_kernel_PSV runs the forward operator with the injected signal
+    read at it = nt-1 ... 0: its final wavefield equals a forward run with the
+    reversed signal, bit for bit."""
+    pb = RC.small_problem(nt=120, fsurf=fsurf, fdorder=fdorder)
+    pb.fwi = RC.full_window(pb)
+    f = O.forward(pb)
+    sig = np.random.default_rng(5).standard_normal((len(pb.rec["x"]), pb.nt)) * 1e-3
+    adj = O.adjoint(pb, pb.rec, sig, f["store"])
+    pb2 = RC.small_problem(nt=120, fsurf=fsurf, fdorder=fdorder)
+    pb2.src, pb2.src_signal, pb2.rec = pb.rec, sig[:, ::-1].copy(), None
+    fw = O.forward(pb2)
+    for k in ("vx", "vz", "sxx", "szx", "szz", "We"):
+        assert np.array_equal(adj[k], fw[k]), k
+
+
+def test_gradient_brute_force_tiny():
+    """Re-evaluates the reference correlation (adjoint_kernel_PSV_GPU.cu:818-863)
+    from independently produced wavefields: the adjoint state before step it is
+    the forward state after nt-1-it steps driven by the reversed residual, the
+    adjoint displacement is the running dt-weighted sum of those velocities.
+    Checks the decimated window, the ordering (displacement updated before the
+    correlation), the v/dt store and every term's sign."""
+    nt = 23
+    pb = RC.small_problem(nx=10, nz=9, npml=3, nt=nt, fsurf=1, nsrc=1, nrec=3, dt=6e-4, fc=60.0)
+    pb.fwi = dict(dt=3, dx=2, dz=2, x1=pb.ppad - 1, x2=pb.nxt - 2, z1=pb.fpad, z2=pb.nzt - 3)
+    nft, nfz, nfx = pb.fwi_dims()
+    f = O.forward(pb)
+    res = np.random.default_rng(7).standard_normal((3, nt))
+    adj = O.adjoint(pb, pb.rec, res, f["store"])
+
+    # forward states before each step (independent runs, no store)
+    fw_states = []
+    for k in range(nt):
+        q = RC.small_problem(nx=10, nz=9, npml=3, nt=k, fsurf=1, nsrc=1, nrec=3, dt=6e-4, fc=60.0) \
+            if k > 0 else None
+        if k == 0:
+            fw_states.append({n: np.zeros((pb.nzt, pb.nxt)) for n in ("vx", "vz", "sxx", "szx", "szz")})
+            continue
+        q.src_signal = pb.src_signal[:, :k].copy()
+        o = O.forward(q)
+        fw_states.append(o)
+    # adjoint states before step it: forward with reversed residual for nt-1-it steps
+    adj_states = {}
+    for it in range(nt - 1, -1, -1):
+        k = nt - 1 - it
+        if k == 0:
+            adj_states[it] = {n: np.zeros((pb.nzt, pb.nxt)) for n in ("vx", "vz", "sxx", "szx", "szz")}
+            continue
+        q = RC.small_problem(nx=10, nz=9, npml=3, nt=k, fsurf=1, nsrc=1, nrec=3, dt=6e-4, fc=60.0)
+        q.src, q.rec = pb.rec, None
+        q.src_signal = res[:, ::-1][:, :k].copy()
+        adj_states[it] = O.forward(q)
+    dt = pb.dt
+    fpad = pb.fpad
+    inner = (slice(fpad, pb.nzt - fpad), slice(fpad, pb.nxt - fpad))
+    ux = np.zeros((pb.nzt, pb.nxt))
+    uz = np.zeros((pb.nzt, pb.nxt))
+    gl = np.zeros((nfz, nfx)); gm = np.zeros((nfz, nfx)); gr = np.zeros((nfz, nfx))
+    lam, mu = pb.lam, pb.mu
+    w = pb.fwi
+    zs = np.arange(w["z1"], w["z2"], w["dz"])
+    xs = np.arange(w["x1"], w["x2"], w["dx"])
+    for it in range(nt - 1, -1, -1):
+        a = adj_states[it]
+        ux[inner] += dt * a["vx"][inner]
+        uz[inner] += dt * a["vz"][inner]
+        if it % w["dt"] == 0:
+            F = fw_states[it]
+            sel = np.ix_(zs, xs)
+            fsxx, fszz, fszx = F["sxx"][sel], F["szz"][sel], F["szx"][sel]
+            fvx, fvz = F["vx"][sel] / dt, F["vz"][sel] / dt
+            L, M = lam[sel], mu[sel]
+            s1 = (fsxx + fszz) * (a["sxx"][sel] + a["szz"][sel]) * 0.25 / (L + M) ** 2
+            s2 = (fsxx - fszz) * (a["sxx"][sel] - a["szz"][sel]) / M ** 2
+            s3 = fszx * a["szx"][sel] / M ** 2
+            s4 = ux[sel] * fvx + uz[sel] * fvz
+            gl[:len(zs), :len(xs)] += w["dt"] * dt * s1
+            gm[:len(zs), :len(xs)] += w["dt"] * dt * (s1 + s2 + s3)
+            gr[:len(zs), :len(xs)] -= w["dt"] * dt * s4
+    for ours, ref in ((adj["grad_lam"], gl), (adj["grad_mu"], gm), (adj["grad_rho"], gr)):
+        assert np.abs(ref).max() > 0
+        assert np.allclose(ours, ref, rtol=1e-10, atol=1e-14 * np.abs(ref).max())
+    # the last decimated row/column is never written (strict < in the window loops)
+    assert nfz > len(zs) or nfx > len(xs)
+    assert np.all(adj["grad_lam"][len(zs):, :] == 0) and np.all(adj["grad_lam"][:, len(xs):] == 0)
