@@ -1,0 +1,117 @@
+// psv_aux.hip -- the small kernels around the time step: FP64 misfit and
+// residual, FP64 shot-sum of the gradients, fill and strided copies.
+#include <hip/hip_runtime.h>
+
+#include "psv_aux.h"
+
+namespace psv {
+
+constexpr int kRedThreads = 256;
+
+// One block per (shot slot, chunk).  Residual d - d_obs is formed in FP64
+// (calculate_l2_adjoint_sources, CUDA_week2/fd_wave_model.cu:972-1002),
+// written back in the plan's type as the shot's adjoint source, and
+// 0.5 dt diff^2 is reduced in FP64 in a fixed order: each block writes one
+// partial, misfit_finish adds the partials of a shot in chunk order.
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void misfit_partial_kernel(
+    const T* syn, const double* obs, T* res, const size_t* off, const size_t* len,
+    int nchunk, double half_dt, double* partial, int* flag)
+{
+    const int slot = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    const size_t n = len[slot], o = off[slot];
+    const size_t per = (n + nchunk - 1) / nchunk;
+    const size_t b = (size_t)chunk * per;
+    const size_t e = b + per < n ? b + per : n;
+    double acc = 0.0;
+    for (size_t i = b + threadIdx.x; i < e; i += kRedThreads) {
+        const double d = (double)syn[o + i] - obs[o + i];
+        res[o + i] = (T)d;
+        acc += half_dt * d * d;
+        if (!(fabs(d) <= 1.0e10)) *flag = 1;  // plain store: every writer stores 1
+    }
+    __shared__ double red[kRedThreads];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void misfit_finish_kernel(const double* partial, int nslots, int nchunk, double* out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double m = 0.0;
+    for (int c = 0; c < nchunk; c++) m += partial[(size_t)s * nchunk + c];
+    out[s] = m;
+}
+
+// sum over shots, in the order given by `order`, of the [3][nfz][nfx] gradients
+template <typename T>
+__global__ void grad_sum_kernel(const T* grad, size_t gst, const int* order, int nshots,
+                                size_t n, double* out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = 0; k < nshots; k++) acc += (double)grad[(size_t)order[k] * gst + i];
+    out[i] = acc;
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, size_t n, T v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += step) p[k] = v;
+}
+
+template <typename T>
+hipError_t launch_misfit(const T* syn, const double* obs, T* res, const size_t* off,
+                         const size_t* len, int nslots, int nchunk, double half_dt,
+                         double* partial, double* out, int* flag, hipStream_t s)
+{
+    hipLaunchKernelGGL((misfit_partial_kernel<T>), dim3(nslots * nchunk), dim3(kRedThreads), 0, s,
+                       syn, obs, res, off, len, nchunk, half_dt, partial, flag);
+    hipLaunchKernelGGL(misfit_finish_kernel, dim3((nslots + 63) / 64), dim3(64), 0, s, partial,
+                       nslots, nchunk, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_grad_sum(const T* grad, size_t gst, const int* order, int nshots, size_t n,
+                           double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL((grad_sum_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       grad, gst, order, nshots, n, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill(T* p, size_t n, T v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template hipError_t launch_misfit<float>(const float*, const double*, float*, const size_t*,
+                                         const size_t*, int, int, double, double*, double*, int*,
+                                         hipStream_t);
+template hipError_t launch_misfit<double>(const double*, const double*, double*, const size_t*,
+                                          const size_t*, int, int, double, double*, double*, int*,
+                                          hipStream_t);
+template hipError_t launch_grad_sum<float>(const float*, size_t, const int*, int, size_t, double*,
+                                           hipStream_t);
+template hipError_t launch_grad_sum<double>(const double*, size_t, const int*, int, size_t,
+                                            double*, hipStream_t);
+template hipError_t launch_fill<float>(float*, size_t, float, hipStream_t);
+template hipError_t launch_fill<double>(double*, size_t, double, hipStream_t);
+template hipError_t launch_fill<int>(int*, size_t, int, hipStream_t);
+
+}  // namespace psv

@@ -1,0 +1,70 @@
+// psv_step.h -- launch parameters of the fused P-SV time-step kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace psv {
+
+constexpr int kTX = 64;        // tile width: one wave64 row
+constexpr int kThreads = 256;  // four waves per workgroup
+constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, 32 CUs each
+
+enum { kForward = 0, kAdjoint = 1 };
+enum { kStore = 1, kEnergy = 2 };
+
+// All full-grid arrays share one row pitch (elements) and are addressed as
+// base + slot * stride + iz * pitch + ix.  Averaged materials use the same
+// pitch (their last row and column are unused).
+template <typename T>
+struct StepParams {
+    int nzt, nxt, pitch;
+    int ppad, npml, tp, fsurf, pnx, pnz;
+    int tiles_x, tiles_z, nslots;
+    int it;              // time index (plus *it_dev when it_dev != nullptr)
+    const int* it_dev;   // device-side time counter for graph replays
+    int flags;           // kStore | kEnergy
+    T dt, dxi, dzi, hc1, hc2;
+
+    // model, shared by every shot of the batch
+    const T *lam, *mu, *mu_zx, *rho_zp, *rho_xp;
+    const T* pml;  // a, b, K, a_half, b_half, K_half, each tp long
+
+    // wavefields: read set 0, write set 1; per-shot stride fst
+    const T *vx0, *vz0, *sxx0, *szx0, *szz0;
+    T *vx1, *vz1, *sxx1, *szx1, *szz1;
+    size_t fst;
+    T* We;         // energy weight, stride fst
+    T *ux, *uz;    // adjoint displacement, stride fst
+
+    // CPML memory: x-type [nzt][tp] (stride xst), z-type [tp][pitch] (stride zst)
+    const T *mvxx0, *mvzx0, *mvxz0, *mvzz0;  // stress side, read set
+    T *mvxx1, *mvzx1, *mvxz1, *mvzz1;        // stress side, write set
+    T *msxxx, *mszxx, *mszzz, *mszxz;        // velocity side, in place
+    size_t xst, zst;
+
+    // forward store [ntf][5][nfz][nfx] per shot (stride sst), first tf held = tf0
+    T* store;
+    size_t sst;
+    int tf0;
+    int fwi_dt, fwi_dx, fwi_dz, fwi_x1, fwi_x2, fwi_z1, fwi_z2, nfz, nfx;
+    T* grad;       // [3][nfz][nfx] per shot (stride gst): lambda, mu, rho
+    size_t gst;
+
+    // per-slot tables
+    const int* shot_nt;      // [nslots]
+    const int* inj_off;      // [nslots+1] offsets into the injection groups
+    const int* inj_grp;      // 5 ints per group: z, x, comp, first entry, count
+    const int* inj_row;      // signal row (within the shot) of each entry
+    const T* sig;            // signals, shot s at sig_off[s], row-major [row][nt_s]
+    const size_t* sig_off;
+    const int* rec_off;      // [nslots+1] offsets into rec_zxc
+    const int* rec_zxc;      // 3 ints per receiver: z, x, comp
+    T* rec;                  // traces, shot s at rec_out_off[s], [nrec_s][nt_s]
+    const size_t* rec_out_off;
+};
+
+template <typename T>
+hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s);
+
+}  // namespace psv

@@ -1,0 +1,1103 @@
+// psvfwi.cpp -- plan, device memory and time loops behind the C ABI (psvfwi.h).
+//
+// A plan owns one HIP stream and every device buffer of a batch of shots on
+// one model.  Shots are kept in "slots" sorted by record length (longest
+// first), so the shots still running at step it are always a prefix of the
+// slots and one launch of the step kernel covers exactly them.
+//
+// Forward (forward_kernel_PSV, CUDA_week2/forward_kernel_PSV_GPU.cu:681-1084):
+//   it = 0 .. ntmax-1, state ping-pong between two buffer sets.
+// Adjoint (adjoint_kernel_PSV, CUDA_week2/adjoint_kernel_PSV_GPU.cu:688-1102):
+//   it = ntmax-1 .. 0 on a zeroed state, residuals injected at the receivers.
+// Gradient storage: either the reference's decimated store of every fwi_dt-th
+// forward state (PSVFWI_STORE_FULL), or checkpoints of the full state
+// (wavefields + CPML memory) every k steps; the adjoint then replays each
+// segment forward into a small segment store just before it needs it
+// (PSVFWI_STORE_CHECKPOINT).  Both give the same gradient bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "psv_aux.h"
+#include "psv_step.h"
+#include "psvfwi.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+psvfwi_status fail(psvfwi_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+#define PSV_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? PSVFWI_E_NOMEM : PSVFWI_E_HIP,       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define PSV_TRY(expr)                     \
+    do {                                  \
+        psvfwi_status s_ = (expr);        \
+        if (s_ != PSVFWI_OK) return s_;   \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        if (p && count == n) return hipSuccess;
+        release();
+        if (count == 0) return hipSuccess;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+struct ShotHost {
+    int nt = 0;
+    std::vector<int> sx, sz, sc;
+    std::vector<double> ssig;  // [nsrc][nt]
+    std::vector<int> rx, rz, rc;
+};
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct psvfwi_plan {
+    virtual ~psvfwi_plan() = default;
+    virtual psvfwi_status set_pml(const double*) = 0;
+    virtual psvfwi_status set_model(const double*, const double*, const double*, const double*,
+                                    const double*) = 0;
+    virtual psvfwi_status set_window(const psvfwi_window_desc*) = 0;
+    virtual psvfwi_status set_store_mode(int, int) = 0;
+    virtual psvfwi_status set_shots(int, const psvfwi_shot_desc*) = 0;
+    virtual psvfwi_status set_tile(int) = 0;
+    virtual psvfwi_status forward(int) = 0;
+    virtual psvfwi_status misfit(const double*, double*) = 0;
+    virtual psvfwi_status set_adjoint_sources(int, const double*) = 0;
+    virtual psvfwi_status set_store(int, const double*) = 0;
+    virtual psvfwi_status adjoint(int) = 0;
+    virtual psvfwi_status get_records(int, double*) const = 0;
+    virtual psvfwi_status get_field(int, int, double*) const = 0;
+    virtual psvfwi_status get_store(int, double*) const = 0;
+    virtual psvfwi_status get_gradient(int, double*, double*, double*) const = 0;
+    virtual psvfwi_status timing(double*, long*) const = 0;
+};
+
+namespace {
+
+enum { kF_VX, kF_VZ, kF_SXX, kF_SZX, kF_SZZ };
+// stress-side CPML memory: 0 mem_vx_x, 1 mem_vz_x (x-type), 2 mem_vx_z, 3 mem_vz_z (z-type)
+// velocity-side: 0 mem_sxx_x, 1 mem_szx_x (x-type), 2 mem_szz_z, 3 mem_szx_z (z-type)
+
+template <typename T>
+struct State {
+    DBuf<T> f[2][5];
+    DBuf<T> ms[2][4];
+    DBuf<T> mv[4];
+};
+
+template <typename T>
+class Plan final : public psvfwi_plan {
+public:
+    explicit Plan(const psvfwi_grid_desc& g) : g_(g)
+    {
+        fpad_ = g.fdorder / 2;
+        ppad_ = g.npml + fpad_;
+        tp_ = 2 * (g.npml + 1);
+        pitch_ = (int)round_up((size_t)g.nxt, 64);
+        fst_ = round_up((size_t)g.nzt * pitch_, 64);
+        xst_ = round_up((size_t)g.nzt * tp_, 64);
+        zst_ = round_up((size_t)tp_ * pitch_, 64);
+        pnx_ = g.nxt - 2 * ppad_ + fpad_ - 1;
+        // bottom strip offset; equals the reference's nzt - 2 ppad - 1 at fpad == 1
+        pnz_ = g.fsurf ? g.nzt - 2 * ppad_ + fpad_ - 2 : g.nzt - 2 * ppad_ + fpad_ - 1;
+        tz_ = 16;
+    }
+
+    ~Plan() override
+    {
+        if (ev0_) (void)hipEventDestroy(ev0_);
+        if (ev1_) (void)hipEventDestroy(ev1_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    psvfwi_status init()
+    {
+        PSV_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        PSV_HIP(hipEventCreate(&ev0_));
+        PSV_HIP(hipEventCreate(&ev1_));
+        PSV_HIP(pml_.alloc((size_t)6 * tp_));
+        return PSVFWI_OK;
+    }
+
+    // ------------------------------------------------------------------ setup
+    psvfwi_status set_pml(const double* prof) override
+    {
+        if (!prof) return fail(PSVFWI_E_ARG, "set_pml: null profiles");
+        std::vector<T> h((size_t)6 * tp_);
+        for (size_t i = 0; i < h.size(); i++) h[i] = (T)prof[i];
+        PSV_HIP(hipMemcpyAsync(pml_.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        have_pml_ = true;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_model(const double* lam, const double* mu, const double* mu_zx,
+                            const double* rho_zp, const double* rho_xp) override
+    {
+        if (!lam || !mu || !mu_zx || !rho_zp || !rho_xp) return fail(PSVFWI_E_ARG, "set_model: null array");
+        const int nzt = g_.nzt, nxt = g_.nxt;
+        DBuf<T>* dst[5] = {&lam_, &mu_, &mu_zx_, &rho_zp_, &rho_xp_};
+        const double* src[5] = {lam, mu, mu_zx, rho_zp, rho_xp};
+        std::vector<T> h((size_t)nzt * pitch_);
+        for (int a = 0; a < 5; a++) {
+            const bool avg = a >= 2;  // [nzt-1][nxt-1]
+            const int rows = avg ? nzt - 1 : nzt, cols = avg ? nxt - 1 : nxt;
+            std::fill(h.begin(), h.end(), (T)0);
+            for (int iz = 0; iz < rows; iz++)
+                for (int ix = 0; ix < cols; ix++)
+                    h[(size_t)iz * pitch_ + ix] = (T)src[a][(size_t)iz * cols + ix];
+            PSV_HIP(dst[a]->alloc(h.size()));
+            PSV_HIP(hipMemcpyAsync(dst[a]->p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+            PSV_HIP(hipStreamSynchronize(stream_));
+        }
+        have_model_ = true;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_window(const psvfwi_window_desc* w) override
+    {
+        if (!w) {
+            win_on_ = false;
+            return PSVFWI_OK;
+        }
+        if (w->dt < 1 || w->dx < 1 || w->dz < 1) return fail(PSVFWI_E_ARG, "set_window: strides must be >= 1");
+        if (w->x1 < 0 || w->z1 < 0 || w->x2 > g_.nxt || w->z2 > g_.nzt || w->x2 < w->x1 || w->z2 < w->z1)
+            return fail(PSVFWI_E_ARG, "set_window: window outside the padded grid");
+        w_ = *w;
+        nfz_ = 1 + (w->z2 - w->z1) / w->dz;
+        nfx_ = 1 + (w->x2 - w->x1) / w->dx;
+        win_on_ = true;
+        fwd_grad_done_ = false;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_store_mode(int mode, int interval) override
+    {
+        if (mode != PSVFWI_STORE_FULL && mode != PSVFWI_STORE_CHECKPOINT)
+            return fail(PSVFWI_E_ARG, "set_store_mode: unknown mode");
+        if (mode == PSVFWI_STORE_CHECKPOINT && interval < 1)
+            return fail(PSVFWI_E_ARG, "set_store_mode: checkpoint interval must be >= 1");
+        store_mode_ = mode;
+        ck_k_ = interval;
+        fwd_grad_done_ = false;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_tile(int tz) override
+    {
+        if (tz == 0) tz = 16;
+        const bool ok = tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4);
+        if (!ok) return fail(PSVFWI_E_ARG, "set_tile: tile height must be 8, 16 or (FP32) 32");
+        tz_ = tz;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_shots(int n, const psvfwi_shot_desc* sd) override
+    {
+        if (n < 1 || !sd) return fail(PSVFWI_E_ARG, "set_shots: need at least one shot");
+        std::vector<ShotHost> shots((size_t)n);
+        for (int i = 0; i < n; i++) {
+            const psvfwi_shot_desc& d = sd[i];
+            if (d.nt < 1 || d.nsrc < 0 || d.nrec < 0) return fail(PSVFWI_E_ARG, "set_shots: bad nt/nsrc/nrec");
+            ShotHost& s = shots[(size_t)i];
+            s.nt = d.nt;
+            for (int k = 0; k < d.nsrc; k++) {
+                if (!in_grid(d.src_z[k], d.src_x[k])) return fail(PSVFWI_E_ARG, "set_shots: source outside the grid");
+                if (d.src_comp[k] != 1 && d.src_comp[k] != 2 && d.src_comp[k] != 4)
+                    return fail(PSVFWI_E_ARG, "set_shots: source component must be 1, 2 or 4");
+                s.sx.push_back(d.src_x[k]);
+                s.sz.push_back(d.src_z[k]);
+                s.sc.push_back(d.src_comp[k]);
+            }
+            if (d.nsrc) s.ssig.assign(d.src_signal, d.src_signal + (size_t)d.nsrc * d.nt);
+            for (int k = 0; k < d.nrec; k++) {
+                if (!in_grid(d.rec_z[k], d.rec_x[k])) return fail(PSVFWI_E_ARG, "set_shots: receiver outside the grid");
+                if (d.rec_comp[k] != 1 && d.rec_comp[k] != 2)
+                    return fail(PSVFWI_E_ARG, "set_shots: receiver component must be 1 or 2");
+                s.rx.push_back(d.rec_x[k]);
+                s.rz.push_back(d.rec_z[k]);
+                s.rc.push_back(d.rec_comp[k]);
+            }
+        }
+        shots_.swap(shots);
+        nslots_ = n;
+        shot_of_slot_.resize((size_t)n);
+        std::iota(shot_of_slot_.begin(), shot_of_slot_.end(), 0);
+        std::stable_sort(shot_of_slot_.begin(), shot_of_slot_.end(),
+                         [&](int a, int b) { return shots_[(size_t)a].nt > shots_[(size_t)b].nt; });
+        slot_of_shot_.resize((size_t)n);
+        for (int s = 0; s < n; s++) slot_of_shot_[(size_t)shot_of_slot_[(size_t)s]] = s;
+        ntmax_ = shots_[(size_t)shot_of_slot_[0]].nt;
+        nt_slot_.resize((size_t)n);
+        for (int s = 0; s < n; s++) nt_slot_[(size_t)s] = shots_[(size_t)shot_of_slot_[(size_t)s]].nt;
+        PSV_TRY(upload_tables());
+        fwd_done_ = fwd_grad_done_ = false;
+        adj_ready_.assign((size_t)n, 0);
+        return PSVFWI_OK;
+    }
+
+    // ------------------------------------------------------------------ runs
+    psvfwi_status forward(int flags) override
+    {
+        PSV_TRY(ready());
+        const bool energy = flags & PSVFWI_ENERGY;
+        const bool grad = (flags & PSVFWI_GRADIENT) != 0;
+        if (grad && !win_on_) return fail(PSVFWI_E_STATE, "forward: PSVFWI_GRADIENT needs a window");
+        PSV_TRY(alloc_state(main_));
+        PSV_TRY(zero_state(main_));
+        PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
+        PSV_HIP(psv::launch_fill<T>(rec_.p, rec_.n, (T)0, stream_));
+        const bool full = grad && store_mode_ == PSVFWI_STORE_FULL;
+        const bool ckpt = grad && store_mode_ == PSVFWI_STORE_CHECKPOINT;
+        if (full) {
+            PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
+            PSV_HIP(psv::launch_fill<T>(store_.p, store_.n, (T)0, stream_));
+        }
+        if (ckpt) PSV_TRY(alloc_checkpoints());
+
+        PSV_HIP(hipEventRecord(ev0_, stream_));
+        launches_ = 0;
+        int cur = 0;
+        for (int it = 0; it < ntmax_; it++) {
+            const int na = active(it);
+            if (ckpt && it % ck_k_ == 0) PSV_TRY(save_checkpoint(it / ck_k_, cur, na));
+            psv::StepParams<T> p = base_params(main_, cur, fwd_tab_, true);
+            p.it = it;
+            p.nslots = na;
+            p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
+            p.tf0 = 0;
+            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
+            launches_++;
+            cur ^= 1;
+        }
+        PSV_HIP(hipEventRecord(ev1_, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        float ms = 0;
+        PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        last_ms_ = ms;
+        fin_.resize((size_t)nslots_);
+        for (int s = 0; s < nslots_; s++) fin_[(size_t)s] = nt_slot_[(size_t)s] % 2;
+        fwd_done_ = true;
+        fwd_grad_done_ = grad;
+        store_loaded_ = full;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status misfit(const double* obs, double* out) override
+    {
+        if (!fwd_done_) return fail(PSVFWI_E_STATE, "misfit: run forward first");
+        if (!obs || !out) return fail(PSVFWI_E_ARG, "misfit: null argument");
+        // observed traces in slot layout, FP64 on the device
+        std::vector<double> h(rec_.n);
+        size_t o = 0;
+        for (int i = 0; i < nslots_; i++) {
+            const int s = slot_of_shot_[(size_t)i];
+            const size_t len = shots_[(size_t)i].rx.size() * (size_t)shots_[(size_t)i].nt;
+            std::memcpy(h.data() + rec_out_off_h_[(size_t)s], obs + o, len * sizeof(double));
+            o += len;
+        }
+        PSV_HIP(obs_.alloc(h.size() ? h.size() : 1));
+        if (!h.empty())
+            PSV_HIP(hipMemcpyAsync(obs_.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        const int nchunk = 64;
+        PSV_HIP(partial_.alloc((size_t)nslots_ * nchunk));
+        PSV_HIP(misfit_.alloc((size_t)nslots_));
+        PSV_HIP(flag_.alloc(1));
+        PSV_HIP(psv::launch_fill<int>(flag_.p, 1, 0, stream_));
+        PSV_HIP(psv::launch_misfit<T>(rec_.p, obs_.p, adj_sig_.p, rec_out_off_.p, rec_len_.p, nslots_,
+                                      nchunk, 0.5 * g_.dt, partial_.p, misfit_.p, flag_.p, stream_));
+        std::vector<double> m((size_t)nslots_);
+        int flag = 0;
+        PSV_HIP(hipMemcpyAsync(m.data(), misfit_.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        PSV_HIP(hipMemcpyAsync(&flag, flag_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        for (int i = 0; i < nslots_; i++) out[i] = m[(size_t)slot_of_shot_[(size_t)i]];
+        std::fill(adj_ready_.begin(), adj_ready_.end(), 1);
+        if (flag) return fail(PSVFWI_E_NONFINITE, "misfit: |residual| > 1e10 (the reference aborts here)");
+        for (int i = 0; i < nslots_; i++)
+            if (!std::isfinite(out[i])) return fail(PSVFWI_E_NONFINITE, "misfit: non-finite misfit");
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_adjoint_sources(int shot, const double* sig) override
+    {
+        if (shot < 0 || shot >= nslots_ || !sig) return fail(PSVFWI_E_ARG, "set_adjoint_sources: bad shot");
+        const int s = slot_of_shot_[(size_t)shot];
+        const size_t len = shots_[(size_t)shot].rx.size() * (size_t)shots_[(size_t)shot].nt;
+        std::vector<T> h(len);
+        for (size_t i = 0; i < len; i++) h[i] = (T)sig[i];
+        if (len)
+            PSV_HIP(hipMemcpyAsync(adj_sig_.p + rec_out_off_h_[(size_t)s], h.data(), len * sizeof(T),
+                                   hipMemcpyHostToDevice, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        adj_ready_[(size_t)shot] = 1;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_store(int shot, const double* st) override
+    {
+        if (shot < 0 || shot >= nslots_ || !st) return fail(PSVFWI_E_ARG, "set_store: bad shot");
+        if (!win_on_) return fail(PSVFWI_E_STATE, "set_store: no window");
+        if (store_mode_ != PSVFWI_STORE_FULL) return fail(PSVFWI_E_STATE, "set_store: needs PSVFWI_STORE_FULL");
+        const int ntf = 1 + (ntmax_ - 1) / w_.dt;
+        if (store_.n != (size_t)nslots_ * ntf * 5 * nfz_ * nfx_) {
+            PSV_TRY(alloc_store(ntf));
+            PSV_HIP(psv::launch_fill<T>(store_.p, store_.n, (T)0, stream_));
+        }
+        const int s = slot_of_shot_[(size_t)shot];
+        const int nft = 1 + (shots_[(size_t)shot].nt - 1) / w_.dt;
+        const size_t plane = (size_t)nfz_ * nfx_;
+        std::vector<T> h(sst_, (T)0);
+        for (int f = 0; f < 5; f++)
+            for (int t = 0; t < nft; t++)
+                for (size_t k = 0; k < plane; k++)
+                    h[((size_t)t * 5 + f) * plane + k] = (T)st[((size_t)f * nft + t) * plane + k];
+        PSV_HIP(hipMemcpyAsync(store_.p + (size_t)s * sst_, h.data(), sst_ * sizeof(T), hipMemcpyHostToDevice, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        store_loaded_ = true;
+        fwd_grad_done_ = true;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status adjoint(int flags) override
+    {
+        PSV_TRY(ready());
+        const bool energy = flags & PSVFWI_ENERGY;
+        const bool grad = (flags & PSVFWI_GRADIENT) != 0;
+        if (grad && !win_on_) return fail(PSVFWI_E_STATE, "adjoint: PSVFWI_GRADIENT needs a window");
+        if (grad && !fwd_grad_done_)
+            return fail(PSVFWI_E_STATE, "adjoint: run forward with PSVFWI_GRADIENT (or set_store) first");
+        for (int i = 0; i < nslots_; i++)
+            if (!adj_ready_[(size_t)i] && !shots_[(size_t)i].rx.empty())
+                return fail(PSVFWI_E_STATE, "adjoint: no adjoint sources (call misfit or set_adjoint_sources)");
+        const bool ckpt = grad && store_mode_ == PSVFWI_STORE_CHECKPOINT;
+        if (grad && !ckpt && !store_loaded_) return fail(PSVFWI_E_STATE, "adjoint: forward store missing");
+
+        PSV_TRY(alloc_state(main_));
+        PSV_TRY(zero_state(main_));
+        PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
+        PSV_HIP(ux_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(uz_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(psv::launch_fill<T>(ux_.p, ux_.n, (T)0, stream_));
+        PSV_HIP(psv::launch_fill<T>(uz_.p, uz_.n, (T)0, stream_));
+        if (grad) {
+            gst_ = (size_t)3 * nfz_ * nfx_;
+            PSV_HIP(grad_.alloc((size_t)nslots_ * gst_));
+            PSV_HIP(psv::launch_fill<T>(grad_.p, grad_.n, (T)0, stream_));
+        }
+        const int aflags = (energy ? psv::kEnergy : 0) | (grad ? psv::kStore : 0);
+
+        PSV_HIP(hipEventRecord(ev0_, stream_));
+        launches_ = 0;
+        int cur = 0;
+        if (!ckpt) {
+            for (int it = ntmax_ - 1; it >= 0; it--) {
+                PSV_TRY(adjoint_step(it, cur, aflags, 0));
+                cur ^= 1;
+            }
+        } else {
+            PSV_TRY(alloc_state(replay_));
+            const int kk = ck_k_;
+            const int nseg = (ntmax_ + kk - 1) / kk;
+            const int seg_cap = (kk + w_.dt - 1) / w_.dt + 1;
+            PSV_TRY(alloc_store(seg_cap));
+            for (int c = nseg - 1; c >= 0; c--) {
+                const int t0 = c * kk, t1 = std::min(t0 + kk, ntmax_);
+                const int tf0 = (t0 + w_.dt - 1) / w_.dt;
+                // replay the segment forward from its checkpoint into the segment store
+                PSV_TRY(restore_checkpoint(c, active(t0)));
+                int rcur = 0;
+                for (int it = t0; it < t1; it++) {
+                    psv::StepParams<T> p = base_params(replay_, rcur, fwd_tab_noreceivers_, true);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = psv::kStore;
+                    p.tf0 = tf0;
+                    p.We = nullptr;
+                    PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
+                    launches_++;
+                    rcur ^= 1;
+                }
+                for (int it = t1 - 1; it >= t0; it--) {
+                    PSV_TRY(adjoint_step(it, cur, aflags, tf0));
+                    cur ^= 1;
+                }
+            }
+        }
+        PSV_HIP(hipEventRecord(ev1_, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        float ms = 0;
+        PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        last_ms_ = ms;
+        fin_.assign((size_t)nslots_, ntmax_ % 2);
+        adj_grad_done_ = grad;
+        return PSVFWI_OK;
+    }
+
+    // ------------------------------------------------------------------ getters
+    psvfwi_status get_records(int shot, double* out) const override
+    {
+        if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_records: bad shot");
+        if (!fwd_done_) return fail(PSVFWI_E_STATE, "get_records: run forward first");
+        const int s = slot_of_shot_[(size_t)shot];
+        const size_t len = shots_[(size_t)shot].rx.size() * (size_t)shots_[(size_t)shot].nt;
+        return download(rec_.p + rec_out_off_h_[(size_t)s], len, out);
+    }
+
+    psvfwi_status get_field(int shot, int field, double* out) const override
+    {
+        if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_field: bad shot");
+        if (fin_.empty()) return fail(PSVFWI_E_STATE, "get_field: nothing has run");
+        const int s = slot_of_shot_[(size_t)shot];
+        const int set = fin_[(size_t)s];
+        const int nzt = g_.nzt, nxt = g_.nxt;
+        if (field >= 0 && field <= 7) {
+            const T* base = nullptr;
+            if (field <= 4) base = main_.f[set][field].p;
+            else if (field == 5) base = We_.p;
+            else if (field == 6) base = ux_.p;
+            else base = uz_.p;
+            if (!base) return fail(PSVFWI_E_STATE, "get_field: field not allocated");
+            std::vector<double> h((size_t)nzt * pitch_);
+            PSV_TRY(download(base + (size_t)s * fst_, h.size(), h.data()));
+            for (int iz = 0; iz < nzt; iz++)
+                std::memcpy(out + (size_t)iz * nxt, h.data() + (size_t)iz * pitch_, (size_t)nxt * sizeof(double));
+            return PSVFWI_OK;
+        }
+        if (field >= 8 && field <= 15) {
+            if (g_.npml <= 0) return fail(PSVFWI_E_ARG, "get_field: no CPML");
+            const int k = field - 8;
+            // 8 mem_vx_x, 9 mem_vz_x (stress side, x), 10 mem_sxx_x, 11 mem_szx_x (velocity side, x)
+            // 12 mem_vx_z, 13 mem_vz_z (stress side, z), 14 mem_szz_z, 15 mem_szx_z (velocity side, z)
+            const T* base = nullptr;
+            bool xtype = k < 4;
+            switch (k) {
+            case 0: base = main_.ms[set][0].p; break;
+            case 1: base = main_.ms[set][1].p; break;
+            case 2: base = main_.mv[0].p; break;
+            case 3: base = main_.mv[1].p; break;
+            case 4: base = main_.ms[set][2].p; break;
+            case 5: base = main_.ms[set][3].p; break;
+            case 6: base = main_.mv[2].p; break;
+            default: base = main_.mv[3].p; break;
+            }
+            if (xtype) return download(base + (size_t)s * xst_, (size_t)nzt * tp_, out);
+            std::vector<double> h((size_t)tp_ * pitch_);
+            PSV_TRY(download(base + (size_t)s * zst_, h.size(), h.data()));
+            for (int r = 0; r < tp_; r++)
+                std::memcpy(out + (size_t)r * nxt, h.data() + (size_t)r * pitch_, (size_t)nxt * sizeof(double));
+            return PSVFWI_OK;
+        }
+        return fail(PSVFWI_E_ARG, "get_field: unknown field");
+    }
+
+    psvfwi_status get_store(int shot, double* out) const override
+    {
+        if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_store: bad shot");
+        if (!store_loaded_ || store_mode_ != PSVFWI_STORE_FULL)
+            return fail(PSVFWI_E_STATE, "get_store: no full forward store held");
+        const int s = slot_of_shot_[(size_t)shot];
+        const int nft = 1 + (shots_[(size_t)shot].nt - 1) / w_.dt;
+        const size_t plane = (size_t)nfz_ * nfx_;
+        std::vector<double> h(sst_);
+        PSV_TRY(download(store_.p + (size_t)s * sst_, sst_, h.data()));
+        for (int f = 0; f < 5; f++)
+            for (int t = 0; t < nft; t++)
+                std::memcpy(out + ((size_t)f * nft + t) * plane, h.data() + ((size_t)t * 5 + f) * plane,
+                            plane * sizeof(double));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status get_gradient(int shot, double* gl, double* gm, double* gr) const override
+    {
+        if (!adj_grad_done_) return fail(PSVFWI_E_STATE, "get_gradient: run adjoint with PSVFWI_GRADIENT first");
+        if (shot < -1 || shot >= nslots_ || !gl || !gm || !gr) return fail(PSVFWI_E_ARG, "get_gradient: bad shot");
+        const size_t plane = (size_t)nfz_ * nfx_;
+        std::vector<double> h(3 * plane);
+        if (shot >= 0) {
+            const int s = slot_of_shot_[(size_t)shot];
+            PSV_TRY(download(grad_.p + (size_t)s * gst_, 3 * plane, h.data()));
+        } else {
+            DBuf<double> sum;
+            DBuf<int> order;
+            PSV_HIP(sum.alloc(3 * plane));
+            PSV_HIP(order.alloc((size_t)nslots_));
+            PSV_HIP(hipMemcpy(order.p, slot_of_shot_.data(), (size_t)nslots_ * sizeof(int), hipMemcpyHostToDevice));
+            PSV_HIP(psv::launch_grad_sum<T>(grad_.p, gst_, order.p, nslots_, 3 * plane, sum.p, stream_));
+            PSV_HIP(hipMemcpyAsync(h.data(), sum.p, 3 * plane * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            PSV_HIP(hipStreamSynchronize(stream_));
+        }
+        std::memcpy(gl, h.data(), plane * sizeof(double));
+        std::memcpy(gm, h.data() + plane, plane * sizeof(double));
+        std::memcpy(gr, h.data() + 2 * plane, plane * sizeof(double));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status timing(double* ms, long* launches) const override
+    {
+        if (ms) *ms = last_ms_;
+        if (launches) *launches = launches_;
+        return PSVFWI_OK;
+    }
+
+private:
+    struct Tables {
+        DBuf<int> inj_off, inj_grp, inj_row, rec_off, rec_zxc;
+        DBuf<T> sig;
+        DBuf<size_t> sig_off;
+    };
+
+    bool in_grid(int z, int x) const { return z >= 0 && z < g_.nzt && x >= 0 && x < g_.nxt; }
+
+    int active(int it) const
+    {
+        int n = 0;
+        while (n < nslots_ && nt_slot_[(size_t)n] > it) n++;
+        return n;
+    }
+
+    psvfwi_status ready() const
+    {
+        if (!have_model_) return fail(PSVFWI_E_STATE, "set_model has not been called");
+        if (g_.npml > 0 && !have_pml_) return fail(PSVFWI_E_STATE, "set_pml has not been called");
+        if (nslots_ < 1) return fail(PSVFWI_E_STATE, "set_shots has not been called");
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status download(const T* src, size_t n, double* out) const
+    {
+        if (n == 0) return PSVFWI_OK;
+        std::vector<T> h(n);
+        PSV_HIP(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        for (size_t i = 0; i < n; i++) out[i] = (double)h[i];
+        return PSVFWI_OK;
+    }
+
+    // injection groups: entries sorted by (z, x, comp), original order kept
+    // inside a group, so co-located sources are added in index order
+    psvfwi_status build_injection(Tables& t, bool adjoint_sources, bool with_signals)
+    {
+        std::vector<int> off(1, 0), grp, row;
+        std::vector<size_t> sig_off;
+        std::vector<T> sig;
+        for (int s = 0; s < nslots_; s++) {
+            const ShotHost& sh = shots_[(size_t)shot_of_slot_[(size_t)s]];
+            const std::vector<int>& X = adjoint_sources ? sh.rx : sh.sx;
+            const std::vector<int>& Z = adjoint_sources ? sh.rz : sh.sz;
+            const std::vector<int>& Cc = adjoint_sources ? sh.rc : sh.sc;
+            std::vector<int> idx(X.size());
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+                if (Z[(size_t)a] != Z[(size_t)b]) return Z[(size_t)a] < Z[(size_t)b];
+                if (X[(size_t)a] != X[(size_t)b]) return X[(size_t)a] < X[(size_t)b];
+                return Cc[(size_t)a] < Cc[(size_t)b];
+            });
+            const int base_row = (int)row.size();
+            size_t k = 0;
+            while (k < idx.size()) {
+                size_t e = k;
+                while (e < idx.size() && Z[(size_t)idx[e]] == Z[(size_t)idx[k]] &&
+                       X[(size_t)idx[e]] == X[(size_t)idx[k]] && Cc[(size_t)idx[e]] == Cc[(size_t)idx[k]])
+                    e++;
+                grp.push_back(Z[(size_t)idx[k]]);
+                grp.push_back(X[(size_t)idx[k]]);
+                grp.push_back(Cc[(size_t)idx[k]]);
+                grp.push_back(base_row + (int)k);
+                grp.push_back((int)(e - k));
+                k = e;
+            }
+            for (size_t q = 0; q < idx.size(); q++) row.push_back(idx[q]);
+            off.push_back((int)(grp.size() / 5));
+            if (with_signals) {
+                sig_off.push_back(sig.size());
+                for (double v : sh.ssig) sig.push_back((T)v);
+            }
+        }
+        // inj_row holds signal rows relative to the shot; the kernel adds sig_off[slot]
+        PSV_HIP(t.inj_off.alloc(off.size()));
+        PSV_HIP(t.inj_grp.alloc(std::max<size_t>(grp.size(), 1)));
+        PSV_HIP(t.inj_row.alloc(std::max<size_t>(row.size(), 1)));
+        PSV_HIP(hipMemcpy(t.inj_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!grp.empty()) PSV_HIP(hipMemcpy(t.inj_grp.p, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!row.empty()) PSV_HIP(hipMemcpy(t.inj_row.p, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (with_signals) {
+            PSV_HIP(t.sig.alloc(std::max<size_t>(sig.size(), 1)));
+            if (!sig.empty()) PSV_HIP(hipMemcpy(t.sig.p, sig.data(), sig.size() * sizeof(T), hipMemcpyHostToDevice));
+            PSV_HIP(t.sig_off.alloc(sig_off.size()));
+            PSV_HIP(hipMemcpy(t.sig_off.p, sig_off.data(), sig_off.size() * sizeof(size_t), hipMemcpyHostToDevice));
+        }
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status build_receivers(Tables& t, bool none)
+    {
+        std::vector<int> off(1, 0), zxc;
+        for (int s = 0; s < nslots_; s++) {
+            const ShotHost& sh = shots_[(size_t)shot_of_slot_[(size_t)s]];
+            if (!none)
+                for (size_t r = 0; r < sh.rx.size(); r++) {
+                    zxc.push_back(sh.rz[r]);
+                    zxc.push_back(sh.rx[r]);
+                    zxc.push_back(sh.rc[r]);
+                }
+            off.push_back((int)(zxc.size() / 3));
+        }
+        PSV_HIP(t.rec_off.alloc(off.size()));
+        PSV_HIP(t.rec_zxc.alloc(std::max<size_t>(zxc.size(), 1)));
+        PSV_HIP(hipMemcpy(t.rec_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!zxc.empty()) PSV_HIP(hipMemcpy(t.rec_zxc.p, zxc.data(), zxc.size() * sizeof(int), hipMemcpyHostToDevice));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status upload_tables()
+    {
+        PSV_TRY(build_injection(fwd_tab_, false, true));
+        PSV_TRY(build_injection(adj_tab_, true, false));
+        PSV_TRY(build_injection(fwd_tab_noreceivers_, false, false));
+        PSV_TRY(build_receivers(fwd_tab_, false));
+        PSV_TRY(build_receivers(adj_tab_, true));
+        PSV_TRY(build_receivers(fwd_tab_noreceivers_, true));
+        PSV_HIP(shot_nt_.alloc((size_t)nslots_));
+        PSV_HIP(hipMemcpy(shot_nt_.p, nt_slot_.data(), (size_t)nslots_ * sizeof(int), hipMemcpyHostToDevice));
+        // receiver traces and adjoint sources share one layout: slot s at rec_out_off[s]
+        rec_out_off_h_.assign((size_t)nslots_, 0);
+        std::vector<size_t> len((size_t)nslots_);
+        size_t tot = 0;
+        for (int s = 0; s < nslots_; s++) {
+            const ShotHost& sh = shots_[(size_t)shot_of_slot_[(size_t)s]];
+            rec_out_off_h_[(size_t)s] = tot;
+            len[(size_t)s] = sh.rx.size() * (size_t)sh.nt;
+            tot += len[(size_t)s];
+        }
+        PSV_HIP(rec_out_off_.alloc((size_t)nslots_));
+        PSV_HIP(rec_len_.alloc((size_t)nslots_));
+        PSV_HIP(hipMemcpy(rec_out_off_.p, rec_out_off_h_.data(), (size_t)nslots_ * sizeof(size_t), hipMemcpyHostToDevice));
+        PSV_HIP(hipMemcpy(rec_len_.p, len.data(), (size_t)nslots_ * sizeof(size_t), hipMemcpyHostToDevice));
+        PSV_HIP(rec_.alloc(std::max<size_t>(tot, 1)));
+        PSV_HIP(adj_sig_.alloc(std::max<size_t>(tot, 1)));
+        PSV_HIP(psv::launch_fill<T>(adj_sig_.p, adj_sig_.n, (T)0, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status alloc_state(State<T>& st)
+    {
+        const size_t nf = (size_t)nslots_ * fst_;
+        for (int s = 0; s < 2; s++)
+            for (int f = 0; f < 5; f++) PSV_HIP(st.f[s][f].alloc(nf));
+        const size_t nx = g_.npml > 0 ? (size_t)nslots_ * xst_ : 1;
+        const size_t nz = g_.npml > 0 ? (size_t)nslots_ * zst_ : 1;
+        for (int s = 0; s < 2; s++) {
+            PSV_HIP(st.ms[s][0].alloc(nx));
+            PSV_HIP(st.ms[s][1].alloc(nx));
+            PSV_HIP(st.ms[s][2].alloc(nz));
+            PSV_HIP(st.ms[s][3].alloc(nz));
+        }
+        PSV_HIP(st.mv[0].alloc(nx));
+        PSV_HIP(st.mv[1].alloc(nx));
+        PSV_HIP(st.mv[2].alloc(nz));
+        PSV_HIP(st.mv[3].alloc(nz));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status zero_state(State<T>& st)
+    {
+        for (int s = 0; s < 2; s++) {
+            for (int f = 0; f < 5; f++) PSV_HIP(psv::launch_fill<T>(st.f[s][f].p, st.f[s][f].n, (T)0, stream_));
+            for (int m = 0; m < 4; m++) PSV_HIP(psv::launch_fill<T>(st.ms[s][m].p, st.ms[s][m].n, (T)0, stream_));
+        }
+        for (int m = 0; m < 4; m++) PSV_HIP(psv::launch_fill<T>(st.mv[m].p, st.mv[m].n, (T)0, stream_));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status alloc_store(int ntf)
+    {
+        sst_ = (size_t)ntf * 5 * nfz_ * nfx_;
+        PSV_HIP(store_.alloc((size_t)nslots_ * sst_));
+        return PSVFWI_OK;
+    }
+
+    // checkpoint c holds, for every slot, the 5 wavefields (current set), the
+    // 4 stress-side CPML arrays (current set) and the 4 velocity-side arrays
+    size_t ck_bytes_per_slot() const
+    {
+        const size_t m = g_.npml > 0 ? 4 * (xst_ + zst_) : 0;
+        return (5 * fst_ + m) * sizeof(T);
+    }
+
+    psvfwi_status alloc_checkpoints()
+    {
+        ncp_ = (ntmax_ + ck_k_ - 1) / ck_k_;
+        PSV_HIP(ckpt_.alloc((size_t)ncp_ * nslots_ * (ck_bytes_per_slot() / sizeof(T))));
+        return PSVFWI_OK;
+    }
+
+    // copy/restore the first na slots: each array is [nslots][stride], so a
+    // prefix of slots is one contiguous range
+    template <bool SAVE>
+    psvfwi_status ck_copy(int c, int set, int na, State<T>& st)
+    {
+        T* base = ckpt_.p + (size_t)c * nslots_ * (ck_bytes_per_slot() / sizeof(T));
+        size_t off = 0;
+        auto one = [&](T* arr, size_t stride) -> psvfwi_status {
+            T* ck = base + off;
+            const size_t bytes = (size_t)na * stride * sizeof(T);
+            if (SAVE) PSV_HIP(hipMemcpyAsync(ck, arr, bytes, hipMemcpyDeviceToDevice, stream_));
+            else PSV_HIP(hipMemcpyAsync(arr, ck, bytes, hipMemcpyDeviceToDevice, stream_));
+            off += (size_t)nslots_ * stride;
+            return PSVFWI_OK;
+        };
+        for (int f = 0; f < 5; f++) PSV_TRY(one(st.f[set][f].p, fst_));
+        if (g_.npml > 0) {
+            PSV_TRY(one(st.ms[set][0].p, xst_));
+            PSV_TRY(one(st.ms[set][1].p, xst_));
+            PSV_TRY(one(st.ms[set][2].p, zst_));
+            PSV_TRY(one(st.ms[set][3].p, zst_));
+            PSV_TRY(one(st.mv[0].p, xst_));
+            PSV_TRY(one(st.mv[1].p, xst_));
+            PSV_TRY(one(st.mv[2].p, zst_));
+            PSV_TRY(one(st.mv[3].p, zst_));
+        }
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status save_checkpoint(int c, int set, int na) { return ck_copy<true>(c, set, na, main_); }
+
+    psvfwi_status restore_checkpoint(int c, int na)
+    {
+        // the replay state starts in set 0; the other set is fully rewritten by the first step
+        return ck_copy<false>(c, 0, na, replay_);
+    }
+
+    psvfwi_status adjoint_step(int it, int cur, int flags, int tf0)
+    {
+        psv::StepParams<T> p = base_params(main_, cur, adj_tab_, false);
+        p.it = it;
+        p.nslots = active(it);
+        p.flags = flags;
+        p.tf0 = tf0;
+        PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kAdjoint, tz_, stream_));
+        launches_++;
+        return PSVFWI_OK;
+    }
+
+    psv::StepParams<T> base_params(State<T>& st, int cur, const Tables& tab, bool fwd_sig)
+    {
+        psv::StepParams<T> p{};
+        p.nzt = g_.nzt;
+        p.nxt = g_.nxt;
+        p.pitch = pitch_;
+        p.ppad = ppad_;
+        p.npml = g_.npml;
+        p.tp = tp_;
+        p.fsurf = g_.fsurf;
+        p.pnx = pnx_;
+        p.pnz = pnz_;
+        p.tiles_x = (g_.nxt + psv::kTX - 1) / psv::kTX;
+        p.tiles_z = (g_.nzt + tz_ - 1) / tz_;
+        p.it_dev = nullptr;
+        p.dt = (T)g_.dt;
+        p.dxi = (T)1.0 / (T)g_.dx;
+        p.dzi = (T)1.0 / (T)g_.dz;
+        p.hc1 = (T)g_.hc[1];
+        p.hc2 = (T)g_.hc[2];
+        p.lam = lam_.p;
+        p.mu = mu_.p;
+        p.mu_zx = mu_zx_.p;
+        p.rho_zp = rho_zp_.p;
+        p.rho_xp = rho_xp_.p;
+        p.pml = pml_.p;
+        const int nxt_set = cur ^ 1;
+        p.vx0 = st.f[cur][kF_VX].p;
+        p.vz0 = st.f[cur][kF_VZ].p;
+        p.sxx0 = st.f[cur][kF_SXX].p;
+        p.szx0 = st.f[cur][kF_SZX].p;
+        p.szz0 = st.f[cur][kF_SZZ].p;
+        p.vx1 = st.f[nxt_set][kF_VX].p;
+        p.vz1 = st.f[nxt_set][kF_VZ].p;
+        p.sxx1 = st.f[nxt_set][kF_SXX].p;
+        p.szx1 = st.f[nxt_set][kF_SZX].p;
+        p.szz1 = st.f[nxt_set][kF_SZZ].p;
+        p.fst = fst_;
+        p.We = We_.p;
+        p.ux = ux_.p;
+        p.uz = uz_.p;
+        p.mvxx0 = st.ms[cur][0].p;
+        p.mvzx0 = st.ms[cur][1].p;
+        p.mvxz0 = st.ms[cur][2].p;
+        p.mvzz0 = st.ms[cur][3].p;
+        p.mvxx1 = st.ms[nxt_set][0].p;
+        p.mvzx1 = st.ms[nxt_set][1].p;
+        p.mvxz1 = st.ms[nxt_set][2].p;
+        p.mvzz1 = st.ms[nxt_set][3].p;
+        p.msxxx = st.mv[0].p;
+        p.mszxx = st.mv[1].p;
+        p.mszzz = st.mv[2].p;
+        p.mszxz = st.mv[3].p;
+        p.xst = xst_;
+        p.zst = zst_;
+        p.store = store_.p;
+        p.sst = sst_;
+        if (win_on_) {
+            p.fwi_dt = w_.dt;
+            p.fwi_dx = w_.dx;
+            p.fwi_dz = w_.dz;
+            p.fwi_x1 = w_.x1;
+            p.fwi_x2 = w_.x2;
+            p.fwi_z1 = w_.z1;
+            p.fwi_z2 = w_.z2;
+            p.nfz = nfz_;
+            p.nfx = nfx_;
+        } else {
+            p.fwi_dt = p.fwi_dx = p.fwi_dz = 1;
+        }
+        p.grad = grad_.p;
+        p.gst = gst_;
+        p.shot_nt = shot_nt_.p;
+        p.inj_off = tab.inj_off.p;
+        p.inj_grp = tab.inj_grp.p;
+        p.inj_row = tab.inj_row.p;
+        if (fwd_sig) {
+            p.sig = fwd_tab_.sig.p;
+            p.sig_off = fwd_tab_.sig_off.p;
+        } else {
+            p.sig = adj_sig_.p;
+            p.sig_off = rec_out_off_.p;
+        }
+        p.rec_off = tab.rec_off.p;
+        p.rec_zxc = tab.rec_zxc.p;
+        p.rec = rec_.p;
+        p.rec_out_off = rec_out_off_.p;
+        return p;
+    }
+
+    psvfwi_grid_desc g_;
+    int fpad_ = 1, ppad_ = 0, tp_ = 2, pitch_ = 0, pnx_ = 0, pnz_ = 0, tz_ = 16;
+    size_t fst_ = 0, xst_ = 0, zst_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+    DBuf<T> lam_, mu_, mu_zx_, rho_zp_, rho_xp_, pml_;
+    bool have_model_ = false, have_pml_ = false;
+
+    bool win_on_ = false;
+    psvfwi_window_desc w_{};
+    int nfz_ = 0, nfx_ = 0;
+    int store_mode_ = PSVFWI_STORE_FULL, ck_k_ = 200, ncp_ = 0;
+
+    std::vector<ShotHost> shots_;
+    std::vector<int> shot_of_slot_, slot_of_shot_, nt_slot_;
+    int nslots_ = 0, ntmax_ = 0;
+
+    State<T> main_, replay_;
+    DBuf<T> We_, ux_, uz_, store_, grad_, rec_, adj_sig_, ckpt_;
+    size_t sst_ = 0, gst_ = 0;
+    Tables fwd_tab_, adj_tab_, fwd_tab_noreceivers_;
+    DBuf<int> shot_nt_, flag_;
+    DBuf<size_t> rec_out_off_, rec_len_;
+    std::vector<size_t> rec_out_off_h_;
+    DBuf<double> obs_, partial_, misfit_;
+
+    std::vector<int> fin_;
+    std::vector<char> adj_ready_;
+    bool fwd_done_ = false, fwd_grad_done_ = false, store_loaded_ = false, adj_grad_done_ = false;
+    double last_ms_ = 0;
+    long launches_ = 0;
+};
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* psvfwi_last_error(void) { return g_err.c_str(); }
+
+psvfwi_status psvfwi_plan_create(const psvfwi_grid_desc* g, psvfwi_precision prec, psvfwi_plan** out)
+{
+    if (!g || !out) return fail(PSVFWI_E_ARG, "plan_create: null argument");
+    *out = nullptr;
+    if (g->fdorder != 2 && g->fdorder != 4)
+        return fail(PSVFWI_E_ORDER, "FD order mismatch: fdorder must be 2 or 4");
+    const int fpad = g->fdorder / 2, ppad = g->npml + fpad;
+    if (g->npml < 0 || g->nzt < 2 * fpad + 1 || g->nxt < 2 * fpad + 1)
+        return fail(PSVFWI_E_ARG, "plan_create: grid smaller than the stencil");
+    if (g->npml > 0 && (g->nxt < 2 * ppad || g->nzt < (g->fsurf ? ppad + fpad : 2 * ppad)))
+        return fail(PSVFWI_E_ARG, "plan_create: CPML wider than the grid");
+    if (!(g->dt > 0) || !(g->dx > 0) || !(g->dz > 0)) return fail(PSVFWI_E_ARG, "plan_create: dt, dx, dz must be > 0");
+    if (prec != PSVFWI_FP32 && prec != PSVFWI_FP64) return fail(PSVFWI_E_ARG, "plan_create: unknown precision");
+    std::unique_ptr<psvfwi_plan> p;
+    if (prec == PSVFWI_FP32) {
+        auto q = std::make_unique<Plan<float>>(*g);
+        PSV_TRY(q->init());
+        p = std::move(q);
+    } else {
+        auto q = std::make_unique<Plan<double>>(*g);
+        PSV_TRY(q->init());
+        p = std::move(q);
+    }
+    *out = p.release();
+    return PSVFWI_OK;
+}
+
+psvfwi_status psvfwi_plan_destroy(psvfwi_plan* plan)
+{
+    delete plan;
+    return PSVFWI_OK;
+}
+
+#define PSV_PLAN(plan) \
+    if (!(plan)) return fail(PSVFWI_E_ARG, "null plan")
+
+psvfwi_status psvfwi_set_pml(psvfwi_plan* plan, const double* profiles)
+{
+    PSV_PLAN(plan);
+    return plan->set_pml(profiles);
+}
+
+psvfwi_status psvfwi_set_model(psvfwi_plan* plan, const double* lam, const double* mu,
+                               const double* mu_zx, const double* rho_zp, const double* rho_xp)
+{
+    PSV_PLAN(plan);
+    return plan->set_model(lam, mu, mu_zx, rho_zp, rho_xp);
+}
+
+psvfwi_status psvfwi_set_window(psvfwi_plan* plan, const psvfwi_window_desc* win)
+{
+    PSV_PLAN(plan);
+    return plan->set_window(win);
+}
+
+psvfwi_status psvfwi_set_store_mode(psvfwi_plan* plan, int mode, int interval)
+{
+    PSV_PLAN(plan);
+    return plan->set_store_mode(mode, interval);
+}
+
+psvfwi_status psvfwi_set_shots(psvfwi_plan* plan, int nshots, const psvfwi_shot_desc* shots)
+{
+    PSV_PLAN(plan);
+    return plan->set_shots(nshots, shots);
+}
+
+psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz)
+{
+    PSV_PLAN(plan);
+    return plan->set_tile(tz);
+}
+
+psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags)
+{
+    PSV_PLAN(plan);
+    return plan->forward(flags);
+}
+
+psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out)
+{
+    PSV_PLAN(plan);
+    return plan->misfit(observed, misfit_out);
+}
+
+psvfwi_status psvfwi_set_adjoint_sources(psvfwi_plan* plan, int shot, const double* signal)
+{
+    PSV_PLAN(plan);
+    return plan->set_adjoint_sources(shot, signal);
+}
+
+psvfwi_status psvfwi_set_store(psvfwi_plan* plan, int shot, const double* store)
+{
+    PSV_PLAN(plan);
+    return plan->set_store(shot, store);
+}
+
+psvfwi_status psvfwi_adjoint(psvfwi_plan* plan, int flags)
+{
+    PSV_PLAN(plan);
+    return plan->adjoint(flags);
+}
+
+psvfwi_status psvfwi_get_records(const psvfwi_plan* plan, int shot, double* out)
+{
+    PSV_PLAN(plan);
+    return plan->get_records(shot, out);
+}
+
+psvfwi_status psvfwi_get_field(const psvfwi_plan* plan, int shot, int field, double* out)
+{
+    PSV_PLAN(plan);
+    return plan->get_field(shot, field, out);
+}
+
+psvfwi_status psvfwi_get_store(const psvfwi_plan* plan, int shot, double* out)
+{
+    PSV_PLAN(plan);
+    return plan->get_store(shot, out);
+}
+
+psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* grad_lam,
+                                  double* grad_mu, double* grad_rho)
+{
+    PSV_PLAN(plan);
+    return plan->get_gradient(shot, grad_lam, grad_mu, grad_rho);
+}
+
+psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches)
+{
+    PSV_PLAN(plan);
+    return plan->timing(ms, launches);
+}
+
+psvfwi_status psvfwi_parabola_step(const double* L2, const double* step, double* opt)
+{
+    if (!L2 || !step || !opt) return fail(PSVFWI_E_ARG, "parabola_step: null argument");
+    // Newton divided differences of the interpolating parabola, FP64
+    const double d01 = (L2[1] - L2[0]) / (step[1] - step[0]);
+    const double d02 = (L2[2] - L2[0]) / (step[2] - step[0]);
+    const double x2 = (d02 - d01) / (step[2] - step[1]);
+    const double x1 = d01 - x2 * (step[0] + step[1]);
+    *opt = -x1 / (2.0 * x2);
+    if (!std::isfinite(*opt)) return fail(PSVFWI_E_NONFINITE, "parabola_step: degenerate parabola");
+    return PSVFWI_OK;
+}
+
+}  // extern "C"

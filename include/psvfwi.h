@@ -1,0 +1,156 @@
+/*
+ * psvfwi.h -- C ABI of the MI355X (gfx950) 2D elastic P-SV FWI hot path.
+ *
+ * This is the drop-in boundary for the reference's propagation entry points
+ * (CUDA_week2/fd_kernel.cuh:12-173).  It covers:
+ *   forward_kernel_PSV  (fd_kernel.cuh:12-40)   -> psvfwi_forward + getters
+ *   adjoint_kernel_PSV  (fd_kernel.cuh:43-70)   -> psvfwi_adjoint + getters
+ *   forward_kernel_PSV_GPU / adjoint_kernel_PSV_GPU (fd_kernel.cuh:75-173):
+ *                        the same two calls; device memory is owned by the plan
+ *   calc_opt_step       (fd_kernel.cuh:72)      -> psvfwi_parabola_step
+ *   calculate_l2_adjoint_sources (CUDA_week2/fd_wave_model.cu:972-1002)
+ *                                               -> psvfwi_misfit (FP64 on the GPU)
+ *
+ * Plain C types only: flat row-major arrays and sizes.  Host arrays are FP64
+ * (real_sim = double, CUDA_week2/globvar.cuh:16) and are converted to the
+ * plan's arithmetic type on upload.  Every call returns a status; nothing
+ * exits the process.  One plan per host thread; a plan owns its HIP stream.
+ *
+ * Index conventions follow the reference: a padded grid of nzt x nxt cells,
+ * fpad = fdorder/2, ppad = npml + fpad, interior [fpad, nzt-fpad) x
+ * [fpad, nxt-fpad) (CUDA_week2/fd_grid.cu:52-64); source and receiver
+ * coordinates are padded-grid indices (after grid_adjustment,
+ * CUDA_week2/fd_seismogram.cu:153-164).
+ */
+#ifndef PSVFWI_H
+#define PSVFWI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum psvfwi_status {
+    PSVFWI_OK = 0,
+    PSVFWI_E_ARG = -1,        /* invalid argument or inconsistent sizes */
+    PSVFWI_E_HIP = -2,        /* HIP runtime error (message in psvfwi_last_error) */
+    PSVFWI_E_ORDER = -3,      /* fdorder other than 2 or 4 */
+    PSVFWI_E_NONFINITE = -4,  /* |residual| > 1e10 or a non-finite misfit */
+    PSVFWI_E_STATE = -5,      /* call out of sequence (e.g. adjoint before forward) */
+    PSVFWI_E_NOMEM = -6       /* device allocation failed */
+} psvfwi_status;
+
+typedef enum psvfwi_precision { PSVFWI_FP32 = 0, PSVFWI_FP64 = 1 } psvfwi_precision;
+
+/* flags of psvfwi_forward / psvfwi_adjoint */
+#define PSVFWI_ENERGY 1   /* accumulate the energy weight We (fd_wave.cuh:14-29) */
+#define PSVFWI_GRADIENT 2 /* forward: keep what the adjoint needs; adjoint: correlate */
+
+/* how the adjoint gets the forward wavefield at the correlation steps */
+#define PSVFWI_STORE_FULL 0   /* decimated store of every fwi_dt-th step (reference) */
+#define PSVFWI_STORE_CHECKPOINT 1 /* full state every k steps, segments recomputed */
+
+typedef struct psvfwi_grid_desc {
+    int nzt, nxt;       /* padded grid (FDGrid::total_grid) */
+    int fdorder;        /* 2 (reference) or 4 */
+    int npml;           /* CPML width; 0 disables the strips */
+    int fsurf;          /* free surface on the top row iz == fpad */
+    double dt, dx, dz;
+    double hc[7];       /* Holberg coefficients (holbergcoeff) */
+} psvfwi_grid_desc;
+
+/* FWI window and decimation (FDFWI, CUDA_week2/fd_fwi.cu:16-61), padded
+ * indices, half-open [x1, x2) x [z1, z2), strides dx, dz, time stride dt. */
+typedef struct psvfwi_window_desc {
+    int dt, dx, dz, x1, x2, z1, z2;
+} psvfwi_window_desc;
+
+/* One shot: its record length, its sources (with signals) and receivers.
+ * comp: 1 = vx, 2 = vz (the reference's only type), 4 = explosive
+ * (sxx and szz; sources only). */
+typedef struct psvfwi_shot_desc {
+    int nt;
+    int nsrc;
+    const int* src_x;
+    const int* src_z;
+    const int* src_comp;
+    const double* src_signal;   /* [nsrc][nt] */
+    int nrec;
+    const int* rec_x;
+    const int* rec_z;
+    const int* rec_comp;
+} psvfwi_shot_desc;
+
+typedef struct psvfwi_plan psvfwi_plan;
+
+/* message of the last failing call on this thread */
+const char* psvfwi_last_error(void);
+
+psvfwi_status psvfwi_plan_create(const psvfwi_grid_desc* grid, psvfwi_precision prec,
+                                 psvfwi_plan** out);
+psvfwi_status psvfwi_plan_destroy(psvfwi_plan* plan);
+
+/* CPML profiles a, b, K, a_half, b_half, K_half, each 2(npml+1) long
+ * (FDCpml::cpml_pro, CUDA_week2/fd_cpml.cu:59-192). */
+psvfwi_status psvfwi_set_pml(psvfwi_plan* plan, const double* profiles);
+
+/* lam, mu: [nzt][nxt]; mu_zx, rho_zp, rho_xp: [nzt-1][nxt-1]
+ * (FDMedium::average_parameters, CUDA_week2/fd_medium.cu:128-159). */
+psvfwi_status psvfwi_set_model(psvfwi_plan* plan, const double* lam, const double* mu,
+                               const double* mu_zx, const double* rho_zp, const double* rho_xp);
+
+/* NULL disables gradient work.  The window must lie inside the grid. */
+psvfwi_status psvfwi_set_window(psvfwi_plan* plan, const psvfwi_window_desc* win);
+
+/* Store policy for gradients: PSVFWI_STORE_FULL, or PSVFWI_STORE_CHECKPOINT
+ * with a checkpoint every `interval` steps. */
+psvfwi_status psvfwi_set_store_mode(psvfwi_plan* plan, int mode, int interval);
+
+/* Replaces the batch of shots.  Shot i keeps index i in every getter. */
+psvfwi_status psvfwi_set_shots(psvfwi_plan* plan, int nshots, const psvfwi_shot_desc* shots);
+
+/* Tile height of the step kernel (8, 16 or 32; FP64 8 or 16); 0 = default. */
+psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
+
+/* Runs all shots forward from a zero state: forward_kernel_PSV. */
+psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags);
+
+/* Residual (synthetic - observed) becomes the adjoint source of each shot;
+ * misfit_out[i] = sum 0.5 dt (d - d_obs)^2 accumulated in FP64 on the GPU.
+ * observed: shots concatenated, shot i as [nrec_i][nt_i]. */
+psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out);
+
+/* Explicit adjoint sources for shot i: [nrec_i][nt_i], injected at the shot's
+ * receivers (the reference passes the receivers as sources,
+ * CUDA_week2/fd_wave_model.cu:848-859). */
+psvfwi_status psvfwi_set_adjoint_sources(psvfwi_plan* plan, int shot, const double* signal);
+
+/* Replaces the forward store of one shot ([5][nft][nfz][nfx]: vx/dt, vz/dt,
+ * sxx, szx, szz as fwi_vx..fwi_szz), for a caller that stored it itself. */
+psvfwi_status psvfwi_set_store(psvfwi_plan* plan, int shot, const double* store);
+
+/* Runs all shots' adjoint from a zero state: adjoint_kernel_PSV. */
+psvfwi_status psvfwi_adjoint(psvfwi_plan* plan, int flags);
+
+/* getters (device -> host, converted to FP64) */
+psvfwi_status psvfwi_get_records(const psvfwi_plan* plan, int shot, double* out);   /* [nrec][nt] */
+/* field: 0 vx, 1 vz, 2 sxx, 3 szx, 4 szz, 5 We, 6 ux, 7 uz -> [nzt][nxt];
+ * 8..15 CPML memory mem_vx_x, mem_vz_x, mem_sxx_x, mem_szx_x ([nzt][2(npml+1)]),
+ * mem_vx_z, mem_vz_z, mem_szz_z, mem_szx_z ([2(npml+1)][nxt]) */
+psvfwi_status psvfwi_get_field(const psvfwi_plan* plan, int shot, int field, double* out);
+psvfwi_status psvfwi_get_store(const psvfwi_plan* plan, int shot, double* out); /* [5][nft][nfz][nfx] */
+/* shot >= 0: that shot's gradient; shot == -1: sum over the batch (FP64 sum in shot order) */
+psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* grad_lam,
+                                  double* grad_mu, double* grad_rho);   /* [nfz][nfx] each */
+
+/* Timing of the last forward/adjoint call (milliseconds, HIP events) and the
+ * number of step-kernel launches it made. */
+psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches);
+
+/* Vertex of the parabola through (step[i], L2[i]), i = 0..2, in FP64:
+ * calc_opt_step (CUDA_week2/fwi_step_length_estimation.cu:226-271). */
+psvfwi_status psvfwi_parabola_step(const double* L2, const double* step, double* opt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
