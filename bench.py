@@ -1,0 +1,303 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X P-SV FWI hot path: forward + adjoint + gradient.
+
+Default workload (the single-GPU configuration the metric is quoted on):
+1000 x 500 cells, dx 5 m, order 4 Holberg, 40-cell CPML, 8 flat layers
+(vp 1800-4500 m/s, vs = vp/1.73, Gardner density) with a seeded 3 % Gaussian
+perturbation, one 12 Hz Ricker vz source, 400 receivers, dt 0.4 ms,
+nt 5000, FP32 kernels, gradient window = the model, time decimation 5.
+One "step" = one forward (with the gradient store) + FP64 misfit/residual
++ adjoint with the fused gradient correlation, all on device-resident
+inputs.  Observed data are produced beforehand from the perturbed model;
+the synthetics come from the unperturbed layers.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4]
+
+Prints one JSON line.  Gcell-updates/s = 2 * interior cells * nt / time
+(forward and adjoint counted, checkpoint recomputation not counted).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s fwd+adj+grad, batched vs per-shot shots/s, % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def make_plan(w, precision, window, store_mode, interval, tile, shots=None):
+    from cuda_fwi_amd import psvfwi as P
+    vp0, vs0, rho0 = w.background
+    lam, mu, rho = w.padded(vp0, vs0, rho0)
+    mu_zx, rho_zp, rho_xp = P.average_parameters(mu, rho)
+    hc = P.holberg(w.fdorder, w.max_relative_error)
+    plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt,
+                  dx=w.dx, dz=w.dx, hc=hc, precision=precision)
+    vmax = float(np.max(w.vp))
+    plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, vmax, w.dt))
+    plan.set_window(window)
+    plan.set_store_mode(store_mode, interval)
+    shots = w.shots if shots is None else shots
+    plan.set_shots([w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in shots])
+    # observed data from the true model, then switch to the starting model
+    lt, mt, rt = w.padded()
+    plan.set_model(lt, mt, *P.average_parameters(mt, rt))
+    plan.forward(energy=False)
+    plan.set_observed([plan.records(i) for i in range(len(shots))])
+    plan.set_model(lam, mu, mu_zx, rho_zp, rho_xp)
+    if tile > 0:
+        plan.set_tile(tile)
+        plan.tile = tile
+    else:
+        plan.tile = plan.autotune(min(200, w.nt))
+    return plan
+
+
+def alg_bytes(w, nwin, fwi_dt, esize):
+    """Algorithmic HBM bytes per launch of the step kernel (per shot):
+    wavefield update 60 B per interior cell (5 fields read+written, 5 model
+    arrays read), CPML memory 32 B per strip cell, adjoint displacement
+    16 B per cell, gradient correlation 44 B per window cell on correlated
+    steps, forward store 20 B per window cell on stored steps (amortised over
+    fwi_dt); FP32 figures scaled by esize / 4."""
+    s = esize / 4.0
+    n = w.interior_cells()
+    rows, cols = w.nzt - 2 * w.fpad, w.nxt - 2 * w.fpad
+    strip = 2 * w.npml * rows + (1 if w.fsurf else 2) * w.npml * cols
+    fwd = s * (60 * n + 32 * strip + 20 * nwin / fwi_dt)
+    adj = s * (76 * n + 32 * strip + 44 * nwin / fwi_dt)
+    return fwd, adj
+
+
+def cpu_baseline(w, window, nt_sample):
+    """The oracle (single-threaded FP64 CPU restatement of the reference path)
+    on the same grid, model and geometry, for nt_sample steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import psv_oracle as O
+    from cuda_fwi_amd import psvfwi as P
+    vp0, vs0, rho0 = w.background
+    lam, mu, rho = w.padded(vp0, vs0, rho0)
+    s = w.shots[0]
+    ps = w.padded_shot(s, P.ricker(nt_sample, w.dt, s["amp"], w.fc)[None])
+    src = dict(ps["src"], shot=[0] * len(ps["src"]["x"]))
+    rec = dict(ps["rec"], shot=[0] * len(ps["rec"]["x"]))
+    pml = O.cpml_pro(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt)
+    pb = O.Problem(nt=nt_sample, nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, dt=w.dt,
+                   dx=w.dx, dz=w.dx, fsurf=w.fsurf, hc=O.holbergcoeff(w.fdorder, w.max_relative_error),
+                   lam=lam, mu=mu, rho=rho, pml=pml, src=src, src_signal=ps["src_signal"], rec=rec,
+                   fwi=window)
+    obs = np.zeros((len(rec["x"]), nt_sample))
+    t0 = time.perf_counter()
+    f = O.forward(pb)
+    _, res, _ = O.l2_adjoint_sources(w.dt, f["rec"], obs)
+    O.adjoint(pb, rec, res, f["store"])
+    t = time.perf_counter() - t0
+    return dict(value=2.0 * w.interior_cells() * nt_sample / t / 1e9, unit="Gcell-updates/s", cores=1,
+                kind="port",
+                sample=f"oracle FP64, 1 thread: same grid/model/geometry, forward+misfit+adjoint+"
+                       f"gradient over nt={nt_sample} of {w.nt} steps ({t:.1f} s)")
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (torch elementwise copy kernel), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copy_ms": ms,
+            "copy_GBps": 2 * n * 4 / ms / 1e6}
+
+
+def run_c2(args):
+    from cuda_fwi_amd import workloads as WL
+    w = WL.c2()
+    if args.nt:
+        w.nt = args.nt
+        w.shots[0]["nt"] = args.nt
+    fwi_dt = args.fwi_dt
+    window = w.model_window(dt=fwi_dt)
+    plan = make_plan(w, args.precision, window, args.store, args.interval, args.tile)
+    _log("setup done (observed data computed)")
+    esize = 4 if args.precision == "fp32" else 8
+
+    def one_pass():
+        plan.forward(energy=False, gradient=True)
+        t_f = plan.timing()
+        plan.misfit()
+        plan.adjoint(energy=False, gradient=True)
+        t_a = plan.timing()
+        return t_f, t_a
+
+    for i in range(args.warmup):
+        one_pass()
+        _log(f"warmup pass {i} done")
+    _sync()
+    t0 = time.perf_counter()
+    tf_ms = ta_ms = 0.0
+    nf = na = 0
+    for i in range(args.steps):
+        (fms, fl), (ams, al) = one_pass()
+        _log(f"timed pass {i}: forward {fms:.1f} ms, adjoint {ams:.1f} ms")
+        tf_ms += fms
+        ta_ms += ams
+        nf += fl
+        na += al
+    _sync()
+    T = time.perf_counter() - t0
+
+    nwin = w.nx * w.nz
+    fwd_b, adj_b = alg_bytes(w, nwin, fwi_dt, esize)
+    # dominant kernel: the one with the larger share of the timed region
+    if ta_ms >= tf_ms:
+        kname, kb, kms, kn = "psv_step_kernel<adjoint>", adj_b, ta_ms, na
+    else:
+        kname, kb, kms, kn = "psv_step_kernel<forward>", fwd_b, tf_ms, nf
+    per_launch_s = kms / 1e3 / kn
+    achieved = kb / per_launch_s / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get(kname.split("<")[1].rstrip(">"))
+    cells = w.interior_cells()
+    value = 2.0 * cells * w.nt * args.steps / T / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(T / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if esize == 4 else "f64",
+        "data": "synthetic: seeded 8-layer model with 3% Gaussian perturbation (observed) and "
+                "the unperturbed layers (synthetic)",
+        "config": {"workload": "1000x500 dx5 order-4 npml40, nt 5000, dt 0.4 ms, 1 shot, "
+                               "400 receivers, fwd+adj+grad",
+                   "padded_grid": [w.nzt, w.nxt], "interior_cells": cells,
+                   "gradient_store": args.store if args.store == "full" else
+                   f"checkpoint every {args.interval}",
+                   "fwi_time_decimation": fwi_dt,
+                   "tile": [64, plan.tile], "tile_choice": "autotuned" if args.tile <= 0 else "fixed"},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "alg_bytes_per_launch": int(kb),
+                     "avg_launch_us": round(per_launch_s * 1e6, 2), "launches": kn},
+        "forward_ms_per_step": round(tf_ms / args.steps, 2),
+        "adjoint_ms_per_step": round(ta_ms / args.steps, 2),
+    }
+    if not args.no_cpu_baseline:
+        _log("cpu baseline (oracle) starting")
+        line["cpu_baseline"] = cpu_baseline(w, window, args.cpu_nt)
+    return line
+
+
+def run_c4(args):
+    """Batched shots (64, uneven nt and receiver counts) in one plan against
+    the same shots run one per plan."""
+    from cuda_fwi_amd import workloads as WL
+    w = WL.c4(nshots=args.nshots)
+    window = w.model_window(dt=args.fwi_dt)
+
+    def fwd_adj(plan):
+        plan.forward(energy=False, gradient=True)
+        plan.misfit()
+        plan.adjoint(energy=False, gradient=True)
+
+    plan = make_plan(w, args.precision, window, "checkpoint", args.interval, args.tile)
+    fwd_adj(plan)  # warmup
+    _sync()
+    t0 = time.perf_counter()
+    fwd_adj(plan)
+    _sync()
+    tb = time.perf_counter() - t0
+    del plan
+    tp = 0.0
+    for s in w.shots[: args.per_shot_sample]:
+        p1 = make_plan(w, args.precision, window, "checkpoint", args.interval, args.tile, shots=[s])
+        _sync()
+        t0 = time.perf_counter()
+        fwd_adj(p1)
+        _sync()
+        tp += time.perf_counter() - t0
+        del p1
+    k = min(args.per_shot_sample, len(w.shots))
+    cell_steps = sum(s["nt"] for s in w.shots) * w.interior_cells()
+    per_shot_rate = k / tp
+    return {
+        "metric": METRIC, "value": round(2.0 * cell_steps / tb / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(tb * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded layered model with a dipping fault",
+        "config": {"workload": f"1500x600 dx5 order-4 npml40, {len(w.shots)} shots, nt U[3000,8000], "
+                               "receivers U[50,400], fwd+adj+grad, checkpoint every "
+                               f"{args.interval}"},
+        "batched_shots_per_s": round(len(w.shots) / tb, 3),
+        "per_shot_shots_per_s": round(per_shot_rate, 3),
+        "per_shot_sample": k,
+        "batched_over_per_shot": round(len(w.shots) / tb / per_shot_rate, 3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--store", default="full", choices=["full", "checkpoint"])
+    ap.add_argument("--interval", type=int, default=200)
+    ap.add_argument("--fwi-dt", type=int, default=5)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--cpu-nt", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nshots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only time a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--per-shot-sample", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(shots shard across GPUs with one all-reduce of the gradient, see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("RANK", "0")) != 0:
+        return
+    if args.calibrate_copy:
+        print(json.dumps(calibrate_copy()), flush=True)
+        return
+    line = run_c2(args) if args.workload == "c2" else run_c4(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

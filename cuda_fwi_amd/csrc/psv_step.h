@@ -15,7 +15,8 @@ enum { kStore = 1, kEnergy = 2 };
 
 // All full-grid arrays share one row pitch (elements) and are addressed as
 // base + slot * stride + iz * pitch + ix.  Averaged materials use the same
-// pitch (their last row and column are unused).
+// pitch (their last row and column are unused).  A step is two launches
+// (stress, then velocity) that update the state in place.
 template <typename T>
 struct StepParams {
     int nzt, nxt, pitch;
@@ -30,17 +31,15 @@ struct StepParams {
     const T *lam, *mu, *mu_zx, *rho_zp, *rho_xp;
     const T* pml;  // a, b, K, a_half, b_half, K_half, each tp long
 
-    // wavefields: read set 0, write set 1; per-shot stride fst
-    const T *vx0, *vz0, *sxx0, *szx0, *szz0;
-    T *vx1, *vz1, *sxx1, *szx1, *szz1;
+    // wavefields, updated in place; per-shot stride fst
+    T *vx, *vz, *sxx, *szx, *szz;
     size_t fst;
     T* We;         // energy weight, stride fst
     T *ux, *uz;    // adjoint displacement, stride fst
 
     // CPML memory: x-type [nzt][tp] (stride xst), z-type [tp][pitch] (stride zst)
-    const T *mvxx0, *mvzx0, *mvxz0, *mvzz0;  // stress side, read set
-    T *mvxx1, *mvzx1, *mvxz1, *mvzz1;        // stress side, write set
-    T *msxxx, *mszxx, *mszzz, *mszxz;        // velocity side, in place
+    T *mvxx, *mvzx, *mvxz, *mvzz;      // stress side: mem_vx_x, mem_vz_x, mem_vx_z, mem_vz_z
+    T *msxxx, *mszxx, *mszzz, *mszxz;  // velocity side: mem_sxx_x, mem_szx_x, mem_szz_z, mem_szx_z
     size_t xst, zst;
 
     // forward store [ntf][5][nfz][nfx] per shot (stride sst), first tf held = tf0
@@ -53,15 +52,22 @@ struct StepParams {
 
     // per-slot tables
     const int* shot_nt;      // [nslots]
-    const int* inj_off;      // [nslots+1] offsets into the injection groups
     const int* inj_grp;      // 5 ints per group: z, x, comp, first entry, count
     const int* inj_row;      // signal row (within the shot) of each entry
     const T* sig;            // signals, shot s at sig_off[s], row-major [row][nt_s]
     const size_t* sig_off;
-    const int* rec_off;      // [nslots+1] offsets into rec_zxc
     const int* rec_zxc;      // 3 ints per receiver: z, x, comp
+    const int* rec_first;    // [nslots] index of the slot's first receiver in rec_zxc
     T* rec;                  // traces, shot s at rec_out_off[s], [nrec_s][nt_s]
     const size_t* rec_out_off;
+    // per-tile lists (CSR over tiles, slot-major: entry slot * ntile + tile)
+    const int* tinj_off;     // [nslots * ntile + 1] into tinj_idx
+    const int* tinj_idx;     // injection group ids
+    const int* trec_off;     // [nslots * ntile + 1] into trec_idx
+    const int* trec_idx;     // receiver ids (global index into rec_zxc)
+    // explosive (stress) source groups: pairs (group id, slot)
+    int nexpl;
+    const int* expl_grp;
 };
 
 template <typename T>

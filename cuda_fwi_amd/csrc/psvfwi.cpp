@@ -98,7 +98,9 @@ struct psvfwi_plan {
     virtual psvfwi_status set_store_mode(int, int) = 0;
     virtual psvfwi_status set_shots(int, const psvfwi_shot_desc*) = 0;
     virtual psvfwi_status set_tile(int) = 0;
+    virtual psvfwi_status autotune(int, int*) = 0;
     virtual psvfwi_status forward(int) = 0;
+    virtual psvfwi_status set_observed(const double*) = 0;
     virtual psvfwi_status misfit(const double*, double*) = 0;
     virtual psvfwi_status set_adjoint_sources(int, const double*) = 0;
     virtual psvfwi_status set_store(int, const double*) = 0;
@@ -118,14 +120,16 @@ enum { kF_VX, kF_VZ, kF_SXX, kF_SZX, kF_SZZ };
 
 template <typename T>
 struct State {
-    DBuf<T> f[2][5];
-    DBuf<T> ms[2][4];
-    DBuf<T> mv[4];
+    DBuf<T> f[5];   // vx, vz, sxx, szx, szz
+    DBuf<T> ms[4];  // stress-side CPML memory
+    DBuf<T> mv[4];  // velocity-side CPML memory
 };
 
 template <typename T>
 class Plan final : public psvfwi_plan {
 public:
+    static constexpr int kDefaultTz = 8;
+
     explicit Plan(const psvfwi_grid_desc& g) : g_(g)
     {
         fpad_ = g.fdorder / 2;
@@ -138,7 +142,7 @@ public:
         pnx_ = g.nxt - 2 * ppad_ + fpad_ - 1;
         // bottom strip offset; equals the reference's nzt - 2 ppad - 1 at fpad == 1
         pnz_ = g.fsurf ? g.nzt - 2 * ppad_ + fpad_ - 2 : g.nzt - 2 * ppad_ + fpad_ - 1;
-        tz_ = 16;
+        tz_ = kDefaultTz;
     }
 
     ~Plan() override
@@ -221,12 +225,56 @@ public:
         return PSVFWI_OK;
     }
 
+    static bool tile_ok(int tz) { return tz == 4 || tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4); }
+
     psvfwi_status set_tile(int tz) override
     {
-        if (tz == 0) tz = 16;
-        const bool ok = tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4);
-        if (!ok) return fail(PSVFWI_E_ARG, "set_tile: tile height must be 8, 16 or (FP32) 32");
+        if (tz == 0) tz = kDefaultTz;
+        if (!tile_ok(tz)) return fail(PSVFWI_E_ARG, "set_tile: tile height must be 4, 8, 16 or (FP32) 32");
         tz_ = tz;
+        return PSVFWI_OK;
+    }
+
+    // Times `steps` forward launches of the current batch for every tile
+    // height and keeps the fastest.  Runs on the plan's own state buffers
+    // (every forward/adjoint starts from a zeroed state, so nothing leaks).
+    psvfwi_status autotune(int steps, int* best) override
+    {
+        PSV_TRY(ready());
+        if (steps < 1) return fail(PSVFWI_E_ARG, "autotune: steps must be >= 1");
+        PSV_TRY(alloc_state(main_));
+        PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
+        const int cand[4] = {4, 8, 16, 32};
+        float best_ms = 1e30f;
+        int best_tz = tz_;
+        for (int c = 0; c < 4; c++) {
+            if (!tile_ok(cand[c])) continue;
+            tz_ = cand[c];
+            PSV_TRY(ensure_all_tile_lists());
+            PSV_TRY(zero_state(main_));
+            for (int rep = 0; rep < 2; rep++) {  // first repetition warms up
+                PSV_HIP(hipEventRecord(ev0_, stream_));
+                for (int it = 0; it < std::min(steps, ntmax_); it++) {
+                    psv::StepParams<T> p = base_params(main_, fwd_tab_noreceivers_, true);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = 0;
+                    PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
+                }
+                PSV_HIP(hipEventRecord(ev1_, stream_));
+                PSV_HIP(hipStreamSynchronize(stream_));
+            }
+            float ms = 0;
+            PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            tune_ms_[c] = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_tz = tz_;
+            }
+        }
+        tz_ = best_tz;
+        if (best) *best = best_tz;
+        ran_ = false;
         return PSVFWI_OK;
     }
 
@@ -269,7 +317,7 @@ public:
         nt_slot_.resize((size_t)n);
         for (int s = 0; s < n; s++) nt_slot_[(size_t)s] = shots_[(size_t)shot_of_slot_[(size_t)s]].nt;
         PSV_TRY(upload_tables());
-        fwd_done_ = fwd_grad_done_ = false;
+        fwd_done_ = fwd_grad_done_ = have_obs_ = false;
         adj_ready_.assign((size_t)n, 0);
         return PSVFWI_OK;
     }
@@ -294,40 +342,38 @@ public:
         }
         if (ckpt) PSV_TRY(alloc_checkpoints());
 
+        PSV_TRY(ensure_all_tile_lists());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
-        int cur = 0;
         for (int it = 0; it < ntmax_; it++) {
             const int na = active(it);
-            if (ckpt && it % ck_k_ == 0) PSV_TRY(save_checkpoint(it / ck_k_, cur, na));
-            psv::StepParams<T> p = base_params(main_, cur, fwd_tab_, true);
+            if (ckpt && it % ck_k_ == 0) PSV_TRY(save_checkpoint(it / ck_k_, na));
+            psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
             p.it = it;
             p.nslots = na;
             p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
             p.tf0 = 0;
             PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
             launches_++;
-            cur ^= 1;
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
-        fin_.resize((size_t)nslots_);
-        for (int s = 0; s < nslots_; s++) fin_[(size_t)s] = nt_slot_[(size_t)s] % 2;
+        ran_ = true;
         fwd_done_ = true;
         fwd_grad_done_ = grad;
         store_loaded_ = full;
         return PSVFWI_OK;
     }
 
-    psvfwi_status misfit(const double* obs, double* out) override
+    psvfwi_status set_observed(const double* obs) override
     {
-        if (!fwd_done_) return fail(PSVFWI_E_STATE, "misfit: run forward first");
-        if (!obs || !out) return fail(PSVFWI_E_ARG, "misfit: null argument");
+        if (nslots_ < 1) return fail(PSVFWI_E_STATE, "set_observed: set_shots first");
+        if (!obs) return fail(PSVFWI_E_ARG, "set_observed: null argument");
         // observed traces in slot layout, FP64 on the device
-        std::vector<double> h(rec_.n);
+        std::vector<double> h(rec_.n, 0.0);
         size_t o = 0;
         for (int i = 0; i < nslots_; i++) {
             const int s = slot_of_shot_[(size_t)i];
@@ -335,9 +381,19 @@ public:
             std::memcpy(h.data() + rec_out_off_h_[(size_t)s], obs + o, len * sizeof(double));
             o += len;
         }
-        PSV_HIP(obs_.alloc(h.size() ? h.size() : 1));
-        if (!h.empty())
-            PSV_HIP(hipMemcpyAsync(obs_.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        PSV_HIP(obs_.alloc(h.size()));
+        PSV_HIP(hipMemcpyAsync(obs_.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        have_obs_ = true;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status misfit(const double* obs, double* out) override
+    {
+        if (!fwd_done_) return fail(PSVFWI_E_STATE, "misfit: run forward first");
+        if (!out) return fail(PSVFWI_E_ARG, "misfit: null argument");
+        if (obs) PSV_TRY(set_observed(obs));
+        if (!have_obs_) return fail(PSVFWI_E_STATE, "misfit: no observed traces");
         const int nchunk = 64;
         PSV_HIP(partial_.alloc((size_t)nslots_ * nchunk));
         PSV_HIP(misfit_.alloc((size_t)nslots_));
@@ -426,15 +482,12 @@ public:
             PSV_HIP(psv::launch_fill<T>(grad_.p, grad_.n, (T)0, stream_));
         }
         const int aflags = (energy ? psv::kEnergy : 0) | (grad ? psv::kStore : 0);
+        PSV_TRY(ensure_all_tile_lists());
 
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
-        int cur = 0;
         if (!ckpt) {
-            for (int it = ntmax_ - 1; it >= 0; it--) {
-                PSV_TRY(adjoint_step(it, cur, aflags, 0));
-                cur ^= 1;
-            }
+            for (int it = ntmax_ - 1; it >= 0; it--) PSV_TRY(adjoint_step(it, aflags, 0));
         } else {
             PSV_TRY(alloc_state(replay_));
             const int kk = ck_k_;
@@ -446,9 +499,8 @@ public:
                 const int tf0 = (t0 + w_.dt - 1) / w_.dt;
                 // replay the segment forward from its checkpoint into the segment store
                 PSV_TRY(restore_checkpoint(c, active(t0)));
-                int rcur = 0;
                 for (int it = t0; it < t1; it++) {
-                    psv::StepParams<T> p = base_params(replay_, rcur, fwd_tab_noreceivers_, true);
+                    psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
                     p.it = it;
                     p.nslots = active(it);
                     p.flags = psv::kStore;
@@ -456,12 +508,8 @@ public:
                     p.We = nullptr;
                     PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
                     launches_++;
-                    rcur ^= 1;
                 }
-                for (int it = t1 - 1; it >= t0; it--) {
-                    PSV_TRY(adjoint_step(it, cur, aflags, tf0));
-                    cur ^= 1;
-                }
+                for (int it = t1 - 1; it >= t0; it--) PSV_TRY(adjoint_step(it, aflags, tf0));
             }
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
@@ -469,7 +517,7 @@ public:
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
-        fin_.assign((size_t)nslots_, ntmax_ % 2);
+        ran_ = true;
         adj_grad_done_ = grad;
         return PSVFWI_OK;
     }
@@ -487,13 +535,12 @@ public:
     psvfwi_status get_field(int shot, int field, double* out) const override
     {
         if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_field: bad shot");
-        if (fin_.empty()) return fail(PSVFWI_E_STATE, "get_field: nothing has run");
+        if (!ran_) return fail(PSVFWI_E_STATE, "get_field: nothing has run");
         const int s = slot_of_shot_[(size_t)shot];
-        const int set = fin_[(size_t)s];
         const int nzt = g_.nzt, nxt = g_.nxt;
         if (field >= 0 && field <= 7) {
             const T* base = nullptr;
-            if (field <= 4) base = main_.f[set][field].p;
+            if (field <= 4) base = main_.f[field].p;
             else if (field == 5) base = We_.p;
             else if (field == 6) base = ux_.p;
             else base = uz_.p;
@@ -512,12 +559,12 @@ public:
             const T* base = nullptr;
             bool xtype = k < 4;
             switch (k) {
-            case 0: base = main_.ms[set][0].p; break;
-            case 1: base = main_.ms[set][1].p; break;
+            case 0: base = main_.ms[0].p; break;
+            case 1: base = main_.ms[1].p; break;
             case 2: base = main_.mv[0].p; break;
             case 3: base = main_.mv[1].p; break;
-            case 4: base = main_.ms[set][2].p; break;
-            case 5: base = main_.ms[set][3].p; break;
+            case 4: base = main_.ms[2].p; break;
+            case 5: base = main_.ms[3].p; break;
             case 6: base = main_.mv[2].p; break;
             default: base = main_.mv[3].p; break;
             }
@@ -582,9 +629,14 @@ public:
 
 private:
     struct Tables {
-        DBuf<int> inj_off, inj_grp, inj_row, rec_off, rec_zxc;
+        DBuf<int> inj_grp, inj_row, rec_zxc, rec_first;
         DBuf<T> sig;
         DBuf<size_t> sig_off;
+        // host copies: per-slot ranges of groups / receivers and their cells
+        std::vector<int> grp_off, grp_zx, rec_off, rec_zx;
+        // per-tile lists for the tile height they were built for
+        DBuf<int> tinj_off, tinj_idx, trec_off, trec_idx;
+        int built_tz = -1;
     };
 
     bool in_grid(int z, int x) const { return z >= 0 && z < g_.nzt && x >= 0 && x < g_.nxt; }
@@ -655,10 +707,17 @@ private:
             }
         }
         // inj_row holds signal rows relative to the shot; the kernel adds sig_off[slot]
-        PSV_HIP(t.inj_off.alloc(off.size()));
+        t.grp_off = off;
+        t.grp_zx.clear();
+        if (with_signals) fwd_grp_comp_.clear();
+        for (size_t q = 0; q < grp.size(); q += 5) {
+            t.grp_zx.push_back(grp[q]);
+            t.grp_zx.push_back(grp[q + 1]);
+            if (with_signals) fwd_grp_comp_.push_back(grp[q + 2]);
+        }
+        t.built_tz = -1;
         PSV_HIP(t.inj_grp.alloc(std::max<size_t>(grp.size(), 1)));
         PSV_HIP(t.inj_row.alloc(std::max<size_t>(row.size(), 1)));
-        PSV_HIP(hipMemcpy(t.inj_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!grp.empty()) PSV_HIP(hipMemcpy(t.inj_grp.p, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!row.empty()) PSV_HIP(hipMemcpy(t.inj_row.p, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
         if (with_signals) {
@@ -673,6 +732,7 @@ private:
     psvfwi_status build_receivers(Tables& t, bool none)
     {
         std::vector<int> off(1, 0), zxc;
+        t.rec_zx.clear();
         for (int s = 0; s < nslots_; s++) {
             const ShotHost& sh = shots_[(size_t)shot_of_slot_[(size_t)s]];
             if (!none)
@@ -680,13 +740,61 @@ private:
                     zxc.push_back(sh.rz[r]);
                     zxc.push_back(sh.rx[r]);
                     zxc.push_back(sh.rc[r]);
+                    t.rec_zx.push_back(sh.rz[r]);
+                    t.rec_zx.push_back(sh.rx[r]);
                 }
             off.push_back((int)(zxc.size() / 3));
         }
-        PSV_HIP(t.rec_off.alloc(off.size()));
+        t.rec_off = off;
+        t.built_tz = -1;
+        PSV_HIP(t.rec_first.alloc(off.size()));
         PSV_HIP(t.rec_zxc.alloc(std::max<size_t>(zxc.size(), 1)));
-        PSV_HIP(hipMemcpy(t.rec_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        PSV_HIP(hipMemcpy(t.rec_first.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!zxc.empty()) PSV_HIP(hipMemcpy(t.rec_zxc.p, zxc.data(), zxc.size() * sizeof(int), hipMemcpyHostToDevice));
+        return PSVFWI_OK;
+    }
+
+    // CSR lists of the injection groups and receivers of every (slot, tile)
+    // for the current tile height, so a workgroup touches only its own
+    psvfwi_status ensure_tile_lists(Tables& t)
+    {
+        if (t.built_tz == tz_) return PSVFWI_OK;
+        const int tiles_x = (g_.nxt + psv::kTX - 1) / psv::kTX;
+        const int tiles_z = (g_.nzt + tz_ - 1) / tz_;
+        const size_t ntile = (size_t)tiles_x * tiles_z;
+        auto build = [&](const std::vector<int>& soff, const std::vector<int>& zx, DBuf<int>& doff,
+                         DBuf<int>& didx) -> psvfwi_status {
+            std::vector<int> cnt((size_t)nslots_ * ntile + 1, 0);
+            std::vector<std::pair<size_t, int>> items;
+            for (int s = 0; s < nslots_; s++)
+                for (int k = soff[(size_t)s]; k < soff[(size_t)s + 1]; k++) {
+                    const int z = zx[2 * (size_t)k], x = zx[2 * (size_t)k + 1];
+                    const size_t tix = (size_t)s * ntile + (size_t)(z / tz_) * tiles_x + (size_t)(x / psv::kTX);
+                    items.emplace_back(tix, k);
+                }
+            std::stable_sort(items.begin(), items.end(),
+                             [](const std::pair<size_t, int>& a, const std::pair<size_t, int>& b) { return a.first < b.first; });
+            for (const auto& it : items) cnt[it.first + 1]++;
+            for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+            std::vector<int> idx;
+            for (const auto& it : items) idx.push_back(it.second);
+            PSV_HIP(doff.alloc(cnt.size()));
+            PSV_HIP(didx.alloc(std::max<size_t>(idx.size(), 1)));
+            PSV_HIP(hipMemcpy(doff.p, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (!idx.empty()) PSV_HIP(hipMemcpy(didx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+            return PSVFWI_OK;
+        };
+        PSV_TRY(build(t.grp_off, t.grp_zx, t.tinj_off, t.tinj_idx));
+        PSV_TRY(build(t.rec_off, t.rec_zx, t.trec_off, t.trec_idx));
+        t.built_tz = tz_;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status ensure_all_tile_lists()
+    {
+        PSV_TRY(ensure_tile_lists(fwd_tab_));
+        PSV_TRY(ensure_tile_lists(adj_tab_));
+        PSV_TRY(ensure_tile_lists(fwd_tab_noreceivers_));
         return PSVFWI_OK;
     }
 
@@ -698,6 +806,19 @@ private:
         PSV_TRY(build_receivers(fwd_tab_, false));
         PSV_TRY(build_receivers(adj_tab_, true));
         PSV_TRY(build_receivers(fwd_tab_noreceivers_, true));
+        {
+            // (group, slot) pairs of the stress (explosive) sources
+            std::vector<int> ex;
+            for (int s = 0; s < nslots_; s++)
+                for (int k = fwd_tab_.grp_off[(size_t)s]; k < fwd_tab_.grp_off[(size_t)s + 1]; k++)
+                    if (fwd_grp_comp_[(size_t)k] == 4) {
+                        ex.push_back(k);
+                        ex.push_back(s);
+                    }
+            nexpl_ = (int)(ex.size() / 2);
+            PSV_HIP(expl_grp_.alloc(std::max<size_t>(ex.size(), 2)));
+            if (!ex.empty()) PSV_HIP(hipMemcpy(expl_grp_.p, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
         PSV_HIP(shot_nt_.alloc((size_t)nslots_));
         PSV_HIP(hipMemcpy(shot_nt_.p, nt_slot_.data(), (size_t)nslots_ * sizeof(int), hipMemcpyHostToDevice));
         // receiver traces and adjoint sources share one layout: slot s at rec_out_off[s]
@@ -724,30 +845,29 @@ private:
     psvfwi_status alloc_state(State<T>& st)
     {
         const size_t nf = (size_t)nslots_ * fst_;
-        for (int s = 0; s < 2; s++)
-            for (int f = 0; f < 5; f++) PSV_HIP(st.f[s][f].alloc(nf));
+        for (int f = 0; f < 5; f++) PSV_HIP(st.f[f].alloc(nf));
         const size_t nx = g_.npml > 0 ? (size_t)nslots_ * xst_ : 1;
         const size_t nz = g_.npml > 0 ? (size_t)nslots_ * zst_ : 1;
-        for (int s = 0; s < 2; s++) {
-            PSV_HIP(st.ms[s][0].alloc(nx));
-            PSV_HIP(st.ms[s][1].alloc(nx));
-            PSV_HIP(st.ms[s][2].alloc(nz));
-            PSV_HIP(st.ms[s][3].alloc(nz));
+        for (State<T>* q : {&st}) {
+            PSV_HIP(q->ms[0].alloc(nx));
+            PSV_HIP(q->ms[1].alloc(nx));
+            PSV_HIP(q->ms[2].alloc(nz));
+            PSV_HIP(q->ms[3].alloc(nz));
+            PSV_HIP(q->mv[0].alloc(nx));
+            PSV_HIP(q->mv[1].alloc(nx));
+            PSV_HIP(q->mv[2].alloc(nz));
+            PSV_HIP(q->mv[3].alloc(nz));
         }
-        PSV_HIP(st.mv[0].alloc(nx));
-        PSV_HIP(st.mv[1].alloc(nx));
-        PSV_HIP(st.mv[2].alloc(nz));
-        PSV_HIP(st.mv[3].alloc(nz));
         return PSVFWI_OK;
     }
 
     psvfwi_status zero_state(State<T>& st)
     {
-        for (int s = 0; s < 2; s++) {
-            for (int f = 0; f < 5; f++) PSV_HIP(psv::launch_fill<T>(st.f[s][f].p, st.f[s][f].n, (T)0, stream_));
-            for (int m = 0; m < 4; m++) PSV_HIP(psv::launch_fill<T>(st.ms[s][m].p, st.ms[s][m].n, (T)0, stream_));
+        for (int f = 0; f < 5; f++) PSV_HIP(psv::launch_fill<T>(st.f[f].p, st.f[f].n, (T)0, stream_));
+        for (int m = 0; m < 4; m++) {
+            PSV_HIP(psv::launch_fill<T>(st.ms[m].p, st.ms[m].n, (T)0, stream_));
+            PSV_HIP(psv::launch_fill<T>(st.mv[m].p, st.mv[m].n, (T)0, stream_));
         }
-        for (int m = 0; m < 4; m++) PSV_HIP(psv::launch_fill<T>(st.mv[m].p, st.mv[m].n, (T)0, stream_));
         return PSVFWI_OK;
     }
 
@@ -776,7 +896,7 @@ private:
     // copy/restore the first na slots: each array is [nslots][stride], so a
     // prefix of slots is one contiguous range
     template <bool SAVE>
-    psvfwi_status ck_copy(int c, int set, int na, State<T>& st)
+    psvfwi_status ck_copy(int c, int na, State<T>& st)
     {
         T* base = ckpt_.p + (size_t)c * nslots_ * (ck_bytes_per_slot() / sizeof(T));
         size_t off = 0;
@@ -788,12 +908,12 @@ private:
             off += (size_t)nslots_ * stride;
             return PSVFWI_OK;
         };
-        for (int f = 0; f < 5; f++) PSV_TRY(one(st.f[set][f].p, fst_));
+        for (int f = 0; f < 5; f++) PSV_TRY(one(st.f[f].p, fst_));
         if (g_.npml > 0) {
-            PSV_TRY(one(st.ms[set][0].p, xst_));
-            PSV_TRY(one(st.ms[set][1].p, xst_));
-            PSV_TRY(one(st.ms[set][2].p, zst_));
-            PSV_TRY(one(st.ms[set][3].p, zst_));
+            PSV_TRY(one(st.ms[0].p, xst_));
+            PSV_TRY(one(st.ms[1].p, xst_));
+            PSV_TRY(one(st.ms[2].p, zst_));
+            PSV_TRY(one(st.ms[3].p, zst_));
             PSV_TRY(one(st.mv[0].p, xst_));
             PSV_TRY(one(st.mv[1].p, xst_));
             PSV_TRY(one(st.mv[2].p, zst_));
@@ -802,17 +922,13 @@ private:
         return PSVFWI_OK;
     }
 
-    psvfwi_status save_checkpoint(int c, int set, int na) { return ck_copy<true>(c, set, na, main_); }
+    psvfwi_status save_checkpoint(int c, int na) { return ck_copy<true>(c, na, main_); }
 
-    psvfwi_status restore_checkpoint(int c, int na)
-    {
-        // the replay state starts in set 0; the other set is fully rewritten by the first step
-        return ck_copy<false>(c, 0, na, replay_);
-    }
+    psvfwi_status restore_checkpoint(int c, int na) { return ck_copy<false>(c, na, replay_); }
 
-    psvfwi_status adjoint_step(int it, int cur, int flags, int tf0)
+    psvfwi_status adjoint_step(int it, int flags, int tf0)
     {
-        psv::StepParams<T> p = base_params(main_, cur, adj_tab_, false);
+        psv::StepParams<T> p = base_params(main_, adj_tab_, false);
         p.it = it;
         p.nslots = active(it);
         p.flags = flags;
@@ -822,7 +938,7 @@ private:
         return PSVFWI_OK;
     }
 
-    psv::StepParams<T> base_params(State<T>& st, int cur, const Tables& tab, bool fwd_sig)
+    psv::StepParams<T> base_params(State<T>& st, const Tables& tab, bool fwd_sig)
     {
         psv::StepParams<T> p{};
         p.nzt = g_.nzt;
@@ -848,29 +964,19 @@ private:
         p.rho_zp = rho_zp_.p;
         p.rho_xp = rho_xp_.p;
         p.pml = pml_.p;
-        const int nxt_set = cur ^ 1;
-        p.vx0 = st.f[cur][kF_VX].p;
-        p.vz0 = st.f[cur][kF_VZ].p;
-        p.sxx0 = st.f[cur][kF_SXX].p;
-        p.szx0 = st.f[cur][kF_SZX].p;
-        p.szz0 = st.f[cur][kF_SZZ].p;
-        p.vx1 = st.f[nxt_set][kF_VX].p;
-        p.vz1 = st.f[nxt_set][kF_VZ].p;
-        p.sxx1 = st.f[nxt_set][kF_SXX].p;
-        p.szx1 = st.f[nxt_set][kF_SZX].p;
-        p.szz1 = st.f[nxt_set][kF_SZZ].p;
+        p.vx = st.f[kF_VX].p;
+        p.vz = st.f[kF_VZ].p;
+        p.sxx = st.f[kF_SXX].p;
+        p.szx = st.f[kF_SZX].p;
+        p.szz = st.f[kF_SZZ].p;
         p.fst = fst_;
         p.We = We_.p;
         p.ux = ux_.p;
         p.uz = uz_.p;
-        p.mvxx0 = st.ms[cur][0].p;
-        p.mvzx0 = st.ms[cur][1].p;
-        p.mvxz0 = st.ms[cur][2].p;
-        p.mvzz0 = st.ms[cur][3].p;
-        p.mvxx1 = st.ms[nxt_set][0].p;
-        p.mvzx1 = st.ms[nxt_set][1].p;
-        p.mvxz1 = st.ms[nxt_set][2].p;
-        p.mvzz1 = st.ms[nxt_set][3].p;
+        p.mvxx = st.ms[0].p;
+        p.mvzx = st.ms[1].p;
+        p.mvxz = st.ms[2].p;
+        p.mvzz = st.ms[3].p;
         p.msxxx = st.mv[0].p;
         p.mszxx = st.mv[1].p;
         p.mszzz = st.mv[2].p;
@@ -895,9 +1001,12 @@ private:
         p.grad = grad_.p;
         p.gst = gst_;
         p.shot_nt = shot_nt_.p;
-        p.inj_off = tab.inj_off.p;
         p.inj_grp = tab.inj_grp.p;
         p.inj_row = tab.inj_row.p;
+        p.tinj_off = tab.tinj_off.p;
+        p.tinj_idx = tab.tinj_idx.p;
+        p.trec_off = tab.trec_off.p;
+        p.trec_idx = tab.trec_idx.p;
         if (fwd_sig) {
             p.sig = fwd_tab_.sig.p;
             p.sig_off = fwd_tab_.sig_off.p;
@@ -905,10 +1014,12 @@ private:
             p.sig = adj_sig_.p;
             p.sig_off = rec_out_off_.p;
         }
-        p.rec_off = tab.rec_off.p;
+        p.rec_first = tab.rec_first.p;
         p.rec_zxc = tab.rec_zxc.p;
         p.rec = rec_.p;
         p.rec_out_off = rec_out_off_.p;
+        p.nexpl = fwd_sig ? nexpl_ : 0;
+        p.expl_grp = expl_grp_.p;
         return p;
     }
 
@@ -939,11 +1050,16 @@ private:
     std::vector<size_t> rec_out_off_h_;
     DBuf<double> obs_, partial_, misfit_;
 
-    std::vector<int> fin_;
+    bool ran_ = false;
     std::vector<char> adj_ready_;
+    int nexpl_ = 0;
+    DBuf<int> expl_grp_;
+    std::vector<int> fwd_grp_comp_;
     bool fwd_done_ = false, fwd_grad_done_ = false, store_loaded_ = false, adj_grad_done_ = false;
+    bool have_obs_ = false;
     double last_ms_ = 0;
     long launches_ = 0;
+    float tune_ms_[4] = {0, 0, 0, 0};
 };
 
 }  // namespace
@@ -1026,10 +1142,22 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz)
     return plan->set_tile(tz);
 }
 
+psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile)
+{
+    PSV_PLAN(plan);
+    return plan->autotune(steps, best_tile);
+}
+
 psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags)
 {
     PSV_PLAN(plan);
     return plan->forward(flags);
+}
+
+psvfwi_status psvfwi_set_observed(psvfwi_plan* plan, const double* observed)
+{
+    PSV_PLAN(plan);
+    return plan->set_observed(observed);
 }
 
 psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out)

@@ -28,10 +28,12 @@ FIELDS = {"vx": 0, "vz": 1, "sxx": 2, "szx": 3, "szz": 4, "We": 5, "ux": 6, "uz"
           "mem_vx_z": 12, "mem_vz_z": 13, "mem_szz_z": 14, "mem_szx_z": 15}
 EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "psvfwi_set_pml",
            "psvfwi_set_model", "psvfwi_set_window", "psvfwi_set_store_mode", "psvfwi_set_shots",
-           "psvfwi_set_tile", "psvfwi_forward", "psvfwi_misfit", "psvfwi_set_adjoint_sources",
+           "psvfwi_set_tile", "psvfwi_autotune", "psvfwi_forward", "psvfwi_set_observed", "psvfwi_misfit",
+           "psvfwi_set_adjoint_sources",
            "psvfwi_set_store", "psvfwi_adjoint", "psvfwi_get_records", "psvfwi_get_field",
            "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing",
-           "psvfwi_parabola_step")
+           "psvfwi_parabola_step", "psvfwi_holberg", "psvfwi_cpml_profiles",
+           "psvfwi_average_parameters", "psvfwi_ricker")
 
 
 class PsvfwiError(RuntimeError):
@@ -141,6 +143,11 @@ class Plan:
     def set_tile(self, tz):
         _check(lib().psvfwi_set_tile(self._h, int(tz)))
 
+    def autotune(self, steps=50):
+        best = C.c_int()
+        _check(lib().psvfwi_autotune(self._h, int(steps), C.byref(best)))
+        return best.value
+
     def set_shots(self, shots):
         """shots: list of dicts nt, src (x, z, comp), src_signal [nsrc, nt], rec (x, z, comp)."""
         keep, descs = [], []
@@ -164,12 +171,24 @@ class Plan:
     def forward(self, energy=True, gradient=False):
         _check(lib().psvfwi_forward(self._h, (ENERGY if energy else 0) | (GRADIENT if gradient else 0)))
 
-    def misfit(self, observed):
-        """observed: list (one [nrec, nt] array per shot).  Returns per-shot misfits."""
-        obs = np.concatenate([np.ascontiguousarray(o, np.float64).ravel() for o in observed]) \
-            if observed else np.zeros(1)
+    @staticmethod
+    def _concat(observed):
+        parts = [np.ascontiguousarray(o, np.float64).ravel() for o in observed]
+        return np.concatenate(parts) if parts and sum(p.size for p in parts) else np.zeros(1)
+
+    def set_observed(self, observed):
+        """observed: list (one [nrec, nt] array per shot), kept on the device."""
+        obs = self._concat(observed)
+        _check(lib().psvfwi_set_observed(self._h, _dptr(obs)))
+
+    def misfit(self, observed=None):
+        """Per-shot misfits; observed as in set_observed, or None for the resident traces."""
         out = np.zeros(len(self._shots))
-        _check(lib().psvfwi_misfit(self._h, _dptr(obs), _dptr(out)))
+        if observed is None:
+            _check(lib().psvfwi_misfit(self._h, None, _dptr(out)))
+        else:
+            obs = self._concat(observed)
+            _check(lib().psvfwi_misfit(self._h, _dptr(obs), _dptr(out)))
         return out
 
     def set_adjoint_sources(self, shot, signal):
@@ -234,3 +253,35 @@ def parabola_step(L2, steps):
     out = C.c_double()
     _check(lib().psvfwi_parabola_step(a, b, C.byref(out)))
     return out.value
+
+
+# ---- host-side setup functions of the library (FP64) ----
+def holberg(fdorder, max_relative_error=1):
+    hc = np.zeros(7)
+    _check(lib().psvfwi_holberg(int(fdorder), int(max_relative_error), _dptr(hc)))
+    return hc
+
+
+def cpml_profiles(npml, dh, fpml, npower, k_max, rcoef, damp_v, dt):
+    out = np.zeros((6, 2 * (npml + 1)))
+    _check(lib().psvfwi_cpml_profiles(int(npml), C.c_double(dh), C.c_double(fpml),
+                                      C.c_double(npower), C.c_double(k_max), C.c_double(rcoef),
+                                      C.c_double(damp_v), C.c_double(dt), _dptr(out)))
+    return out
+
+
+def average_parameters(mu, rho):
+    mu = np.ascontiguousarray(mu, np.float64)
+    rho = np.ascontiguousarray(rho, np.float64)
+    nzt, nxt = mu.shape
+    out = [np.zeros((nzt - 1, nxt - 1)) for _ in range(3)]
+    _check(lib().psvfwi_average_parameters(int(nzt), int(nxt), _dptr(mu), _dptr(rho),
+                                           *[_dptr(o) for o in out]))
+    return tuple(out)  # mu_zx, rho_zp, rho_xp
+
+
+def ricker(nt, dt, amp, fc, ts=0.0):
+    s = np.zeros(int(nt))
+    _check(lib().psvfwi_ricker(int(nt), C.c_double(dt), C.c_double(amp), C.c_double(fc),
+                               C.c_double(ts), _dptr(s)))
+    return s

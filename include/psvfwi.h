@@ -108,15 +108,26 @@ psvfwi_status psvfwi_set_store_mode(psvfwi_plan* plan, int mode, int interval);
 /* Replaces the batch of shots.  Shot i keeps index i in every getter. */
 psvfwi_status psvfwi_set_shots(psvfwi_plan* plan, int nshots, const psvfwi_shot_desc* shots);
 
-/* Tile height of the step kernel (8, 16 or 32; FP64 8 or 16); 0 = default. */
+/* Tile height of the step kernel (4, 8, 16 or 32; FP64 4, 8 or 16); 0 = default (8). */
 psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
+
+/* Autotuner: times `steps` forward steps of the current batch with every
+ * tile height, keeps the fastest and returns it in *best_tile (may be NULL).
+ * Needs set_model / set_pml / set_shots first.  (No counterpart in the
+ * reference, whose launch shape is fixed at 16 x 16,
+ * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */
+psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile);
 
 /* Runs all shots forward from a zero state: forward_kernel_PSV. */
 psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags);
 
+/* Observed traces kept on the device in FP64: shots concatenated, shot i as
+ * [nrec_i][nt_i] (the reference's signal_meas, CUDA_week2/fd_seismogram.cu:127-131). */
+psvfwi_status psvfwi_set_observed(psvfwi_plan* plan, const double* observed);
+
 /* Residual (synthetic - observed) becomes the adjoint source of each shot;
  * misfit_out[i] = sum 0.5 dt (d - d_obs)^2 accumulated in FP64 on the GPU.
- * observed: shots concatenated, shot i as [nrec_i][nt_i]. */
+ * observed: as in psvfwi_set_observed, or NULL to use the traces set there. */
 psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out);
 
 /* Explicit adjoint sources for shot i: [nrec_i][nt_i], injected at the shot's
@@ -145,6 +156,25 @@ psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* gra
 /* Timing of the last forward/adjoint call (milliseconds, HIP events) and the
  * number of step-kernel launches it made. */
 psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches);
+
+/* ---- host-side setup (FP64): the inputs the kernels read ---- */
+
+/* hc[0..6] for FD order 2..12 and error class 0..4: holbergcoeff
+ * (CUDA_week2/holbergcoeff.cu:6-65) */
+psvfwi_status psvfwi_holberg(int fdorder, int max_relative_error, double* hc);
+
+/* a, b, K, a_half, b_half, K_half into profiles[6 * 2(npml+1)]:
+ * FDCpml::cpml_pro (CUDA_week2/fd_cpml.cu:59-192) */
+psvfwi_status psvfwi_cpml_profiles(int npml, double dh, double fpml, double npower, double k_max,
+                                   double rcoef, double damp_v, double dt, double* profiles);
+
+/* mu_zx, rho_zp, rho_xp [nzt-1][nxt-1] from mu, rho [nzt][nxt]:
+ * FDMedium::average_parameters (CUDA_week2/fd_medium.cu:128-159) */
+psvfwi_status psvfwi_average_parameters(int nzt, int nxt, const double* mu, const double* rho,
+                                        double* mu_zx, double* rho_zp, double* rho_xp);
+
+/* Ricker wavelet: create_wavelet shape 1 (CUDA_week2/fd_seismogram.cu:166-186) */
+psvfwi_status psvfwi_ricker(int nt, double dt, double amp, double fc, double ts, double* signal);
 
 /* Vertex of the parabola through (step[i], L2[i]), i = 0..2, in FP64:
  * calc_opt_step (CUDA_week2/fwi_step_length_estimation.cu:226-271). */

@@ -41,3 +41,26 @@ def test_parabola_step_is_host_only_and_matches_polyfit():
     assert abs(P.parabola_step(L2, sl) - 0.012) < 1e-12
     with pytest.raises(P.PsvfwiError):
         P.parabola_step([1.0, 1.0, 1.0], [0.0, 1.0, 2.0])
+
+
+def test_library_setup_functions_equal_the_oracle():
+    """The product's host setup (holbergcoeff, cpml_pro, average_parameters,
+    create_wavelet) reproduces the oracle bit for bit."""
+    import numpy as np
+    import psv_oracle as O
+    from cuda_fwi_amd import psvfwi as P
+    for order in (2, 4, 6, 8, 10, 12):
+        for err in range(5):
+            assert np.array_equal(P.holberg(order, err), O.holbergcoeff(order, err))
+    for args in [(10, 4.0, 17.0, 2.0, 1.0, 1e-3, 3300.0, 6e-4), (40, 5.0, 12.0, 2.0, 1.0, 1e-3, 4500.0, 4e-4),
+                 (7, 2.5, 30.0, 3.0, 2.5, 1e-4, 2000.0, 1e-4)]:
+        assert np.array_equal(P.cpml_profiles(*args), O.cpml_pro(*args))
+    rng = np.random.default_rng(0)
+    mu = rng.uniform(1e9, 4e9, (33, 47))
+    rho = rng.uniform(1000, 3000, (33, 47))
+    mu[4, 5] = 0.0
+    rho[10, 10:13] = 1e-5
+    rho[11, 10] = 1e-5
+    for a, b in zip(P.average_parameters(mu, rho), O.average_parameters(mu, rho)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(P.ricker(3000, 4e-4, 2.0, 12.0, 0.01), O.create_wavelet(3000, 4e-4, 2.0, 12.0, 0.01))

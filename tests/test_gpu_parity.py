@@ -159,7 +159,7 @@ def test_checkpoint_gradient_equals_full_store(prec):
             assert np.array_equal(a, b), k
 
 
-@pytest.mark.parametrize("tz", [8, 32])
+@pytest.mark.parametrize("tz", [4, 16, 32])
 def test_tile_heights_agree(tz):
     print(f"tile {tz}", flush=True)
     pb = RC.small_problem(nx=100, nz=70, npml=10, nt=200, fdorder=4, fsurf=1)
@@ -170,6 +170,17 @@ def test_tile_heights_agree(tz):
     assert np.array_equal(a.records(), b.records())
     for k in FIELDS:
         assert np.array_equal(a.field(k), b.field(k))
+
+
+def test_autotune_picks_a_tile_and_keeps_results():
+    print("autotune", flush=True)
+    pb = RC.small_problem(nx=120, nz=60, npml=10, nt=150, fdorder=4, fsurf=0)
+    ref = O.forward(pb, prec="f32")
+    plan = plan_of(pb, "fp32")
+    best = plan.autotune(30)
+    assert best in (4, 8, 16, 32)
+    plan.forward()
+    assert np.array_equal(plan.records(), ref["rec"].astype(np.float64))
 
 
 def test_batched_uneven_shots_equal_single_shots():
