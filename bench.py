@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s fwd+adj+grad, batched vs per-shot shots/s, % HBM peak"
+KERNEL_NAMES = {"forward_stress": "psv_stress_kernel (forward)",
+                "forward_velocity": "psv_velocity_kernel (forward)",
+                "adjoint_stress": "psv_stress_kernel (adjoint: displacement + gradient correlation)",
+                "adjoint_velocity": "psv_velocity_kernel (adjoint)"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -75,19 +79,31 @@ def make_plan(w, precision, window, store_mode, interval, tile, shots=None):
 
 
 def alg_bytes(w, nwin, fwi_dt, esize):
-    """Algorithmic HBM bytes per launch of the step kernel (per shot):
-    wavefield update 60 B per interior cell (5 fields read+written, 5 model
-    arrays read), CPML memory 32 B per strip cell, adjoint displacement
-    16 B per cell, gradient correlation 44 B per window cell on correlated
-    steps, forward store 20 B per window cell on stored steps (amortised over
-    fwi_dt); FP32 figures scaled by esize / 4."""
+    """Algorithmic HBM bytes per launch of each step kernel (one shot).
+
+    The whole-step figures are the 60 B per interior cell of a wavefield
+    update (5 fields read and written: 40 B, 5 model arrays read: 20 B),
+    32 B per CPML strip cell (4 memory arrays read and written), 16 B per
+    cell of adjoint displacement, 44 B per window cell per correlated step
+    and 20 B per window cell per stored step (amortised over fwi_dt).  They
+    are split between the two kernels of a step by what each one updates:
+    stress kernel = stresses read+written (24) + lambda, mu, mu_zx (12) =
+    36 B/cell, half of the CPML memory (16 B/strip cell), the displacement,
+    the correlation and the store; velocity kernel = velocities read+written
+    (16) + the two densities (8) = 24 B/cell and the other 16 B/strip cell.
+    The reads of the other field group (velocities in the stress kernel,
+    stresses in the velocity kernel) are not algorithmic bytes.  FP32
+    figures, scaled by esize / 4."""
     s = esize / 4.0
     n = w.interior_cells()
     rows, cols = w.nzt - 2 * w.fpad, w.nxt - 2 * w.fpad
     strip = 2 * w.npml * rows + (1 if w.fsurf else 2) * w.npml * cols
-    fwd = s * (60 * n + 32 * strip + 20 * nwin / fwi_dt)
-    adj = s * (76 * n + 32 * strip + 44 * nwin / fwi_dt)
-    return fwd, adj
+    return {
+        "forward_stress": s * (36 * n + 16 * strip + 20 * nwin / fwi_dt),
+        "forward_velocity": s * (24 * n + 16 * strip),
+        "adjoint_stress": s * (52 * n + 16 * strip + 44 * nwin / fwi_dt),
+        "adjoint_velocity": s * (24 * n + 16 * strip),
+    }
 
 
 def cpu_baseline(w, window, nt_sample):
@@ -154,10 +170,10 @@ def run_c2(args):
 
     def one_pass():
         plan.forward(energy=False, gradient=True)
-        t_f = plan.timing()
+        t_f = plan.timing() + plan.kernel_timing("forward")
         plan.misfit()
         plan.adjoint(energy=False, gradient=True)
-        t_a = plan.timing()
+        t_a = plan.timing() + plan.kernel_timing("adjoint")
         return t_f, t_a
 
     for i in range(args.warmup):
@@ -167,29 +183,45 @@ def run_c2(args):
     t0 = time.perf_counter()
     tf_ms = ta_ms = 0.0
     nf = na = 0
+    ksum = {"forward_stress": [], "forward_velocity": [], "adjoint_stress": [], "adjoint_velocity": []}
     for i in range(args.steps):
-        (fms, fl), (ams, al) = one_pass()
-        _log(f"timed pass {i}: forward {fms:.1f} ms, adjoint {ams:.1f} ms")
+        (fms, fl, fs_us, fv_us, _), (ams, al, as_us, av_us, _) = one_pass()
+        _log(f"timed pass {i}: forward {fms:.1f} ms, adjoint {ams:.1f} ms; stress/velocity kernel "
+             f"us: forward {fs_us:.2f}/{fv_us:.2f}, adjoint {as_us:.2f}/{av_us:.2f}")
         tf_ms += fms
         ta_ms += ams
         nf += fl
         na += al
+        ksum["forward_stress"].append(fs_us)
+        ksum["forward_velocity"].append(fv_us)
+        ksum["adjoint_stress"].append(as_us)
+        ksum["adjoint_velocity"].append(av_us)
     _sync()
     T = time.perf_counter() - t0
 
     nwin = w.nx * w.nz
-    fwd_b, adj_b = alg_bytes(w, nwin, fwi_dt, esize)
-    # dominant kernel: the one with the larger share of the timed region
-    if ta_ms >= tf_ms:
-        kname, kb, kms, kn = "psv_step_kernel<adjoint>", adj_b, ta_ms, na
-    else:
-        kname, kb, kms, kn = "psv_step_kernel<forward>", fwd_b, tf_ms, nf
-    per_launch_s = kms / 1e3 / kn
+    kb_all = alg_bytes(w, nwin, fwi_dt, esize)
+    # Per-kernel average duration: the loop's time per step (HIP events around
+    # the whole time loop, on the plan's stream) split between the two kernels
+    # of a step in the ratio of their sampled event-bracketed durations (a
+    # single bracketed kernel also carries the event's own dispatch overhead).
+    sampled = {k: float(np.mean(v)) for k, v in ksum.items()}
+    kus = {}
+    for kind, ms, n in (("forward", tf_ms, nf), ("adjoint", ta_ms, na)):
+        step_us = ms * 1e3 / n
+        a, b = sampled[kind + "_stress"], sampled[kind + "_velocity"]
+        kus[kind + "_stress"] = step_us * a / (a + b)
+        kus[kind + "_velocity"] = step_us * b / (a + b)
+    launches = {"forward_stress": nf, "forward_velocity": nf, "adjoint_stress": na, "adjoint_velocity": na}
+    # dominant kernel: the largest share of the timed region (mean duration x launches)
+    kname = max(kus, key=lambda k: kus[k] * launches[k])
+    kb = kb_all[kname]
+    per_launch_s = kus[kname] / 1e6
     achieved = kb / per_launch_s / 1e9
     traffic = None
     prof = os.path.join(ROOT, "profiles", "pmc_traffic_c2.json")
     if os.path.exists(prof):
-        traffic = json.load(open(prof)).get(kname.split("<")[1].rstrip(">"))
+        traffic = json.load(open(prof)).get(kname)
     cells = w.interior_cells()
     value = 2.0 * cells * w.nt * args.steps / T / 1e9
     line = {
@@ -206,10 +238,14 @@ def run_c2(args):
                    f"checkpoint every {args.interval}",
                    "fwi_time_decimation": fwi_dt,
                    "tile": [64, plan.tile], "tile_choice": "autotuned" if args.tile <= 0 else "fixed"},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "alg_bytes_per_launch": int(kb),
-                     "avg_launch_us": round(per_launch_s * 1e6, 2), "launches": kn},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[kname],
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": int(kb), "avg_launch_us": round(kus[kname], 2),
+                     "launches": launches[kname],
+                     "kernel_us": {k: round(v, 2) for k, v in kus.items()},
+                     "step_alg_GBps": round(60.0 * (esize / 4) * cells * w.nt * 2 * args.steps
+                                            / (tf_ms + ta_ms) * 1e3 / 1e9, 1)},
         "forward_ms_per_step": round(tf_ms / args.steps, 2),
         "adjoint_ms_per_step": round(ta_ms / args.steps, 2),
     }

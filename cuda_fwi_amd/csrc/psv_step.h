@@ -70,7 +70,11 @@ struct StepParams {
     const int* expl_grp;
 };
 
+// Launches one step (stress kernel, velocity kernel, and the stress-source
+// kernel when the batch has explosive sources).  e_mid, when not null, is
+// recorded between the stress and the velocity kernel.
 template <typename T>
-hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s);
+hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s,
+                       hipEvent_t e_mid = nullptr);
 
 }  // namespace psv

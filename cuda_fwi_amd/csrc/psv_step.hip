@@ -532,10 +532,14 @@ __global__ void psv_stress_source_kernel(const StepParams<T> p)
 }
 
 template <typename T, int ORDER, int MODE, int TZ>
-static hipError_t launch_one(const StepParams<T>& p, hipStream_t s)
+static hipError_t launch_one(const StepParams<T>& p, hipStream_t s, hipEvent_t e_mid)
 {
     const int blocks = p.tiles_x * p.tiles_z * p.nslots;
     hipLaunchKernelGGL((psv_stress_kernel<T, ORDER, MODE, TZ>), dim3(blocks), dim3(kThreads), 0, s, p);
+    if (e_mid) {
+        const hipError_t e = hipEventRecord(e_mid, s);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((psv_velocity_kernel<T, ORDER, TZ>), dim3(blocks), dim3(kThreads), 0, s, p);
     if (p.nexpl > 0)
         hipLaunchKernelGGL((psv_stress_source_kernel<T>), dim3((p.nexpl + 63) / 64), dim3(64), 0, s, p);
@@ -543,10 +547,11 @@ static hipError_t launch_one(const StepParams<T>& p, hipStream_t s)
 }
 
 #define PSV_CASE(O, M, Z) \
-    if (order == O && mode == M && tz == Z) return launch_one<T, O, M, Z>(p, s);
+    if (order == O && mode == M && tz == Z) return launch_one<T, O, M, Z>(p, s, e_mid);
 
 template <typename T>
-hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s)
+hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s,
+                       hipEvent_t e_mid)
 {
     PSV_CASE(2, kForward, 4) PSV_CASE(2, kAdjoint, 4)
     PSV_CASE(4, kForward, 4) PSV_CASE(4, kAdjoint, 4)
@@ -560,7 +565,7 @@ hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipS
 }
 #undef PSV_CASE
 
-template hipError_t launch_step<float>(const StepParams<float>&, int, int, int, hipStream_t);
-template hipError_t launch_step<double>(const StepParams<double>&, int, int, int, hipStream_t);
+template hipError_t launch_step<float>(const StepParams<float>&, int, int, int, hipStream_t, hipEvent_t);
+template hipError_t launch_step<double>(const StepParams<double>&, int, int, int, hipStream_t, hipEvent_t);
 
 }  // namespace psv

@@ -110,6 +110,7 @@ struct psvfwi_plan {
     virtual psvfwi_status get_store(int, double*) const = 0;
     virtual psvfwi_status get_gradient(int, double*, double*, double*) const = 0;
     virtual psvfwi_status timing(double*, long*) const = 0;
+    virtual psvfwi_status kernel_timing(int, double*, double*, int*) const = 0;
 };
 
 namespace {
@@ -147,6 +148,9 @@ public:
 
     ~Plan() override
     {
+        for (int i = 0; i < kMaxSamples; i++)
+            for (int j = 0; j < 3; j++)
+                if (sample_ev_[i][j]) (void)hipEventDestroy(sample_ev_[i][j]);
         if (ev0_) (void)hipEventDestroy(ev0_);
         if (ev1_) (void)hipEventDestroy(ev1_);
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -343,6 +347,7 @@ public:
         if (ckpt) PSV_TRY(alloc_checkpoints());
 
         PSV_TRY(ensure_all_tile_lists());
+        PSV_TRY(begin_samples());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
         for (int it = 0; it < ntmax_; it++) {
@@ -353,14 +358,14 @@ public:
             p.nslots = na;
             p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
             p.tf0 = 0;
-            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
-            launches_++;
+            PSV_TRY(launch_timed(p, psv::kForward, 0));
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
+        PSV_TRY(collect_samples());
         ran_ = true;
         fwd_done_ = true;
         fwd_grad_done_ = grad;
@@ -484,6 +489,7 @@ public:
         const int aflags = (energy ? psv::kEnergy : 0) | (grad ? psv::kStore : 0);
         PSV_TRY(ensure_all_tile_lists());
 
+        PSV_TRY(begin_samples());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
         if (!ckpt) {
@@ -506,8 +512,7 @@ public:
                     p.flags = psv::kStore;
                     p.tf0 = tf0;
                     p.We = nullptr;
-                    PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
-                    launches_++;
+                    PSV_TRY(launch_timed(p, psv::kForward, 2));
                 }
                 for (int it = t1 - 1; it >= t0; it--) PSV_TRY(adjoint_step(it, aflags, tf0));
             }
@@ -517,6 +522,7 @@ public:
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
+        PSV_TRY(collect_samples());
         ran_ = true;
         adj_grad_done_ = grad;
         return PSVFWI_OK;
@@ -933,8 +939,60 @@ private:
         p.nslots = active(it);
         p.flags = flags;
         p.tf0 = tf0;
-        PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kAdjoint, tz_, stream_));
+        PSV_TRY(launch_timed(p, psv::kAdjoint, 1));
+        return PSVFWI_OK;
+    }
+
+    // Launches one step; on every kSampleEvery-th step of a timed loop the
+    // two kernels are bracketed by events (kind 0 forward, 1 adjoint,
+    // 2 checkpoint replay), read back by collect_samples().
+    static constexpr int kSampleEvery = 16;
+    static constexpr int kMaxSamples = 512;
+
+    psvfwi_status launch_timed(const psv::StepParams<T>& p, int mode, int kind)
+    {
+        const bool sample = sample_ev_[0][0] && (launches_ % kSampleEvery == 0) && nsamp_ < kMaxSamples;
+        if (sample) {
+            PSV_HIP(hipEventRecord(sample_ev_[nsamp_][0], stream_));
+            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, mode, tz_, stream_, sample_ev_[nsamp_][1]));
+            PSV_HIP(hipEventRecord(sample_ev_[nsamp_][2], stream_));
+            samp_kind_[nsamp_] = kind;
+            nsamp_++;
+        } else {
+            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, mode, tz_, stream_));
+        }
         launches_++;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status begin_samples()
+    {
+        if (!sample_ev_[0][0])
+            for (int i = 0; i < kMaxSamples; i++)
+                for (int j = 0; j < 3; j++) PSV_HIP(hipEventCreate(&sample_ev_[i][j]));
+        nsamp_ = 0;
+        return PSVFWI_OK;
+    }
+
+    // after the stream has drained: mean microseconds of each kernel per kind
+    psvfwi_status collect_samples()
+    {
+        for (int k = 0; k < 3; k++) {
+            double a = 0, b = 0;
+            int n = 0;
+            for (int i = 0; i < nsamp_; i++) {
+                if (samp_kind_[i] != k) continue;
+                float m1 = 0, m2 = 0;
+                PSV_HIP(hipEventElapsedTime(&m1, sample_ev_[i][0], sample_ev_[i][1]));
+                PSV_HIP(hipEventElapsedTime(&m2, sample_ev_[i][1], sample_ev_[i][2]));
+                a += m1;
+                b += m2;
+                n++;
+            }
+            kus_[k][0] = n ? a / n * 1e3 : 0;
+            kus_[k][1] = n ? b / n * 1e3 : 0;
+            kn_[k] = n;
+        }
         return PSVFWI_OK;
     }
 
@@ -1060,6 +1118,21 @@ private:
     double last_ms_ = 0;
     long launches_ = 0;
     float tune_ms_[4] = {0, 0, 0, 0};
+    hipEvent_t sample_ev_[kMaxSamples][3] = {};
+    int samp_kind_[kMaxSamples] = {};
+    int nsamp_ = 0;
+    double kus_[3][2] = {};
+    int kn_[3] = {};
+
+public:
+    psvfwi_status kernel_timing(int kind, double* us_stress, double* us_velocity, int* samples) const override
+    {
+        if (kind < 0 || kind > 2) return fail(PSVFWI_E_ARG, "kernel_timing: kind must be 0, 1 or 2");
+        if (us_stress) *us_stress = kus_[kind][0];
+        if (us_velocity) *us_velocity = kus_[kind][1];
+        if (samples) *samples = kn_[kind];
+        return PSVFWI_OK;
+    }
 };
 
 }  // namespace
@@ -1213,6 +1286,13 @@ psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* laun
 {
     PSV_PLAN(plan);
     return plan->timing(ms, launches);
+}
+
+psvfwi_status psvfwi_kernel_timing(const psvfwi_plan* plan, int kind, double* us_stress,
+                                   double* us_velocity, int* samples)
+{
+    PSV_PLAN(plan);
+    return plan->kernel_timing(kind, us_stress, us_velocity, samples);
 }
 
 psvfwi_status psvfwi_parabola_step(const double* L2, const double* step, double* opt)

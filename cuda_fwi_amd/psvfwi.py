@@ -31,7 +31,7 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_set_tile", "psvfwi_autotune", "psvfwi_forward", "psvfwi_set_observed", "psvfwi_misfit",
            "psvfwi_set_adjoint_sources",
            "psvfwi_set_store", "psvfwi_adjoint", "psvfwi_get_records", "psvfwi_get_field",
-           "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing",
+           "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing", "psvfwi_kernel_timing",
            "psvfwi_parabola_step", "psvfwi_holberg", "psvfwi_cpml_profiles",
            "psvfwi_average_parameters", "psvfwi_ricker")
 
@@ -245,6 +245,14 @@ class Plan:
         ms, n = C.c_double(), C.c_long()
         _check(lib().psvfwi_last_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def kernel_timing(self, kind):
+        """Mean us of the stress and velocity kernels sampled in the last run:
+        kind 'forward', 'adjoint' or 'replay'."""
+        k = {"forward": 0, "adjoint": 1, "replay": 2}[kind]
+        a, b, n = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().psvfwi_kernel_timing(self._h, k, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
 
 
 def parabola_step(L2, steps):

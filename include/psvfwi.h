@@ -157,6 +157,15 @@ psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* gra
  * number of step-kernel launches it made. */
 psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches);
 
+/* Per-kernel durations sampled (HIP events on the plan's stream, every 16th
+ * step) during the last forward / adjoint call: kind 0 forward steps, 1
+ * adjoint steps, 2 checkpoint replay steps; mean microseconds of the stress
+ * kernel and of the velocity kernel, and the number of samples.  (A
+ * measurement hook with no counterpart in the reference, which only prints
+ * clock() totals, CUDA_week2/fd_wave_model.cu:593-595.) */
+psvfwi_status psvfwi_kernel_timing(const psvfwi_plan* plan, int kind, double* us_stress,
+                                   double* us_velocity, int* samples);
+
 /* ---- host-side setup (FP64): the inputs the kernels read ---- */
 
 /* hc[0..6] for FD order 2..12 and error class 0..4: holbergcoeff
