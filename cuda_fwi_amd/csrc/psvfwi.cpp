@@ -98,6 +98,9 @@ struct psvfwi_plan {
     virtual psvfwi_status set_store_mode(int, int) = 0;
     virtual psvfwi_status set_shots(int, const psvfwi_shot_desc*) = 0;
     virtual psvfwi_status set_tile(int) = 0;
+    virtual psvfwi_status set_snapshots(int, int) = 0;
+    virtual psvfwi_status get_snapshot(int, int, double*) const = 0;
+    virtual psvfwi_status snapshot_count(int, int*) const = 0;
     virtual psvfwi_status autotune(int, int*) = 0;
     virtual psvfwi_status forward(int) = 0;
     virtual psvfwi_status set_observed(const double*) = 0;
@@ -231,6 +234,34 @@ public:
 
     static bool tile_ok(int tz) { return tz == 4 || tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4); }
 
+    psvfwi_status set_snapshots(int interval, int field) override
+    {
+        if (interval < 0 || field < 0 || field > 4) return fail(PSVFWI_E_ARG, "set_snapshots: bad interval or field");
+        snap_interval_ = interval;
+        snap_field_ = field;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status get_snapshot(int shot, int k, double* out) const override
+    {
+        if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_snapshot: bad shot");
+        const int s = slot_of_shot_[(size_t)shot];
+        const int n = nsnap_slot(s);
+        if (k < 0 || k >= n) return fail(PSVFWI_E_ARG, "get_snapshot: no such snapshot");
+        std::vector<double> h((size_t)g_.nzt * pitch_);
+        PSV_TRY(download(snaps_.p + ((size_t)k * nslots_ + s) * fst_, h.size(), h.data()));
+        for (int iz = 0; iz < g_.nzt; iz++)
+            std::memcpy(out + (size_t)iz * g_.nxt, h.data() + (size_t)iz * pitch_, (size_t)g_.nxt * sizeof(double));
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status snapshot_count(int shot, int* n) const override
+    {
+        if (shot < 0 || shot >= nslots_ || !n) return fail(PSVFWI_E_ARG, "snapshot_count: bad shot");
+        *n = snaps_taken_ ? nsnap_slot(slot_of_shot_[(size_t)shot]) : 0;
+        return PSVFWI_OK;
+    }
+
     psvfwi_status set_tile(int tz) override
     {
         if (tz == 0) tz = kDefaultTz;
@@ -345,6 +376,9 @@ public:
             PSV_HIP(psv::launch_fill<T>(store_.p, store_.n, (T)0, stream_));
         }
         if (ckpt) PSV_TRY(alloc_checkpoints());
+        const bool snap = snap_interval_ > 0;
+        if (snap) PSV_HIP(snaps_.alloc((size_t)((ntmax_ - 1) / snap_interval_ + 1) * nslots_ * fst_));
+        snaps_taken_ = snap;
 
         PSV_TRY(ensure_all_tile_lists());
         PSV_TRY(begin_samples());
@@ -359,6 +393,12 @@ public:
             p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
             p.tf0 = 0;
             PSV_TRY(launch_timed(p, psv::kForward, 0));
+            if (snap && it % snap_interval_ == 0) {
+                // forward_kernel_PSV_GPU.cu:1067-1080: after the step, every snap_interval
+                const size_t k = (size_t)(it / snap_interval_);
+                PSV_HIP(hipMemcpyAsync(snaps_.p + k * nslots_ * fst_, main_.f[snap_field_].p,
+                                       (size_t)na * fst_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
+            }
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
@@ -1112,6 +1152,10 @@ private:
     std::vector<char> adj_ready_;
     int nexpl_ = 0;
     DBuf<int> expl_grp_;
+    int snap_interval_ = 0, snap_field_ = 4;
+    bool snaps_taken_ = false;
+    DBuf<T> snaps_;
+    int nsnap_slot(int s) const { return snap_interval_ > 0 ? (nt_slot_[(size_t)s] - 1) / snap_interval_ + 1 : 0; }
     std::vector<int> fwd_grp_comp_;
     bool fwd_done_ = false, fwd_grad_done_ = false, store_loaded_ = false, adj_grad_done_ = false;
     bool have_obs_ = false;
@@ -1219,6 +1263,24 @@ psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile)
 {
     PSV_PLAN(plan);
     return plan->autotune(steps, best_tile);
+}
+
+psvfwi_status psvfwi_set_snapshots(psvfwi_plan* plan, int interval, int field)
+{
+    PSV_PLAN(plan);
+    return plan->set_snapshots(interval, field);
+}
+
+psvfwi_status psvfwi_snapshot_count(const psvfwi_plan* plan, int shot, int* count)
+{
+    PSV_PLAN(plan);
+    return plan->snapshot_count(shot, count);
+}
+
+psvfwi_status psvfwi_get_snapshot(const psvfwi_plan* plan, int shot, int k, double* out)
+{
+    PSV_PLAN(plan);
+    return plan->get_snapshot(shot, k, out);
 }
 
 psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags)

@@ -33,7 +33,8 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_set_store", "psvfwi_adjoint", "psvfwi_get_records", "psvfwi_get_field",
            "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing", "psvfwi_kernel_timing",
            "psvfwi_parabola_step", "psvfwi_holberg", "psvfwi_cpml_profiles",
-           "psvfwi_average_parameters", "psvfwi_ricker")
+           "psvfwi_average_parameters", "psvfwi_ricker", "psvfwi_set_snapshots",
+           "psvfwi_snapshot_count", "psvfwi_get_snapshot")
 
 
 class PsvfwiError(RuntimeError):
@@ -167,6 +168,17 @@ class Plan:
         _check(lib().psvfwi_set_shots(self._h, len(descs), arr))
         self._shots = [dict(nt=int(s["nt"]), nrec=len((s.get("rec") or {"x": []})["x"]))
                        for s in shots]
+
+    def set_snapshots(self, interval, field="szz"):
+        _check(lib().psvfwi_set_snapshots(self._h, int(interval), FIELDS[field] if isinstance(field, str) else int(field)))
+
+    def snapshots(self, shot=0):
+        n = C.c_int()
+        _check(lib().psvfwi_snapshot_count(self._h, int(shot), C.byref(n)))
+        out = np.zeros((n.value, self.grid["nzt"], self.grid["nxt"]))
+        for k in range(n.value):
+            _check(lib().psvfwi_get_snapshot(self._h, int(shot), k, _dptr(out[k])))
+        return out
 
     def forward(self, energy=True, gradient=False):
         _check(lib().psvfwi_forward(self._h, (ENERGY if energy else 0) | (GRADIENT if gradient else 0)))

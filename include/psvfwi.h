@@ -118,6 +118,14 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
  * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */
 psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile);
 
+/* Snapshots of one wavefield (field 0 vx, 1 vz, 2 sxx, 3 szx, 4 szz) kept on
+ * the device after every step it with it % interval == 0 of the next forward
+ * runs (interval 0 switches them off).  The reference writes szz there
+ * (CUDA_week2/forward_kernel_PSV_GPU.cu:1067-1080). */
+psvfwi_status psvfwi_set_snapshots(psvfwi_plan* plan, int interval, int field);
+psvfwi_status psvfwi_snapshot_count(const psvfwi_plan* plan, int shot, int* count);
+psvfwi_status psvfwi_get_snapshot(const psvfwi_plan* plan, int shot, int k, double* out); /* [nzt][nxt] */
+
 /* Runs all shots forward from a zero state: forward_kernel_PSV. */
 psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags);
 

@@ -249,3 +249,24 @@ def test_colocated_sources_and_boundary_geometry():
     assert np.array_equal(plan.records(), ref["rec"])
     for k in FIELDS:
         assert np.array_equal(plan.field(k), ref[k])
+
+
+def test_snapshots_match_reference_output(golden):
+    """Snapshots every 5 steps of vz (the field the shipped vz_snap<k>.csv
+    hold): FP64 snapshot k equals the oracle after 5k+1 steps bit for bit,
+    and matches the reference's CSV to its 6 printed digits."""
+    print("snapshots fp64 vz / 5", flush=True)
+    from test_oracle_golden import SIX_DIGITS, _match_6_digits
+    pb = RC.reference_run()
+    plan = plan_of(pb, "fp64")
+    plan.set_snapshots(5, "vz")
+    plan.forward(energy=False)
+    snaps = plan.snapshots(0)
+    assert snaps.shape == ((pb.nt - 1) // 5 + 1, pb.nzt, pb.nxt)
+    rows, cols = golden["snap_rows"], golden["snap_cols"]
+    for k in (39, 52, 66):
+        assert _match_6_digits(snaps[k][np.ix_(rows, cols)], golden[f"snap{k}"]) <= SIX_DIGITS
+        strips = snaps[k][:, np.r_[0:12, 611:623]]
+        assert _match_6_digits(strips, golden[f"snap{k}_strips"]) <= SIX_DIGITS
+    ref = O.forward(RC.reference_run(nt=5 * 52 + 1))
+    assert np.array_equal(snaps[52], ref["vz"])
