@@ -33,8 +33,13 @@
  *                            differences of the misfit (sign/term checks)
  *   psvo_l2_adjoint_sources  closed form
  *   psvo_calc_opt_step       numpy.polyfit vertex
- *   psvo_scale_gradients_energy, psvo_fwi_grad_update   parity unpinned
- *                            beyond closed-form checks on tiny inputs
+ *   psvo_scale_gradients_energy  special cases: uniform energies (weight
+ *                            w(1 + 0.005)), skipped off-stride / out-of-loop
+ *                            cells, a single hot cell setting the maximum
+ *   psvo_fwi_grad_update     special cases: the signed grad_max scan (a
+ *                            negative maximum raises its cell by exactly
+ *                            step max(mat); a later entry replaces it), the
+ *                            material bounds, override, the loop's index mix
  */
 #ifndef PSV_ORACLE_H
 #define PSV_ORACLE_H

@@ -131,6 +131,68 @@ def test_parabola_vertex_matches_polyfit():
         assert np.isclose(O.calc_opt_step(L2, sl), -p[1] / (2 * p[0]), rtol=1e-8)
 
 
+def test_energy_scaling_special_cases():
+    """scale_gradients_with_energy_weights (CUDA_week2/fd_wave_model.cu:1004-1052):
+    with Ws = Wr = w everywhere the weight is w(1 + 0.005), so each gradient
+    is divided by w(1.005) C^2; cells outside the loop range [z1, nfz) x
+    [x1, nfx) are untouched; Ws Wr = 4 at one cell and 1 elsewhere gives
+    weights 2.01 and 1.01 there (max sqrt = 2); off-stride cells are skipped."""
+    nfz, nfx = 6, 8
+    W = np.full((9, 11), 3.0)
+    g = [np.full((nfz, nfx), v) for v in (1.0, 2.0, 4.0)]
+    out = O.scale_gradients_energy(W, W, 2.0, 3.0, 5.0, nfz, nfx, 1, 2, 1, 1, g)
+    w = 3.0 + 0.005 * 3.0
+    for o, v, C in zip(out, (1.0, 2.0, 4.0), (3.0, 5.0, 2.0)):
+        assert np.allclose(o[1:, 2:], v / (w * C * C), rtol=1e-15)
+        assert np.all(o[0, :] == v) and np.all(o[:, :2] == v)
+    Ws = np.ones((9, 11))
+    Ws[3, 4] = 4.0
+    out = O.scale_gradients_energy(Ws, np.ones((9, 11)), 1.0, 1.0, 1.0, nfz, nfx, 0, 0, 2, 2,
+                                   [np.ones((nfz, nfx))] * 3)
+    # (3, 4) is off the stride-2 loop: its energy is not seen, (2, 3) is not scaled
+    assert np.isclose(out[0][2, 2], 1 / 1.005) and out[0][2, 3] == 1.0
+    Ws[2, 2] = 4.0
+    out = O.scale_gradients_energy(Ws, np.ones((9, 11)), 1.0, 1.0, 1.0, nfz, nfx, 0, 0, 2, 2,
+                                   [np.ones((nfz, nfx))] * 3)
+    assert np.isclose(out[0][2, 2], 1 / 2.01) and np.isclose(out[0][0, 0], 1 / 1.01)
+
+
+def test_material_update_special_cases():
+    """fwi_grad_update (CUDA_week2/fd_medium.cu:161-240): s = step max(mat) /
+    grad_max, where the scan keeps the signed value of an entry whose
+    magnitude exceeds the current grad_max (so once grad_max is negative the
+    next entry replaces it); with the largest-magnitude entry negative and
+    scanned last, the cell of that entry moves by exactly step max(mat) in the
+    +s grad direction (it rises); results are clipped to the material bounds;
+    without override the old model is kept."""
+    mat = np.full((5, 6), 2.0e9)
+    mat[1, 1] = 4.0e9
+    grad = np.zeros((4, 5))
+    grad[3, 4] = -8.0                      # largest magnitude, negative, scanned last
+    grad[1, 2] = 2.0
+    old, new = O.fwi_grad_update(mat, grad, 0.01, 4, 5, 0, 0, 1, 1, False, 2)
+    assert np.array_equal(old, mat)
+    s = 0.01 * 4.0e9 / -8.0
+    assert new[3, 4] == 2.0e9 + s * -8.0 and new[3, 4] > 2.0e9
+    assert new[1, 2] == 2.0e9 + s * 2.0 and new[1, 2] < 2.0e9
+    g2 = grad.copy()
+    g2[3, 4], g2[2, 3] = 0.0, -8.0         # a later zero entry then becomes grad_max
+    _, bad = O.fwi_grad_update(mat, g2, 0.01, 4, 5, 0, 0, 1, 1, False, 2)
+    assert bad[2, 3] == 0.0 and not np.isfinite(bad[0, 0])   # s = inf: -inf clipped, 0 * inf
+    assert new[4, 5] == mat[4, 5]           # outside the loop range: unchanged
+    old, new = O.fwi_grad_update(mat, grad, 100.0, 4, 5, 0, 0, 1, 1, True, 2)
+    assert new[3, 4] == 48.0e9 and new[1, 2] == 0.0   # clipped to [0, 48e9]
+    assert np.array_equal(old[:4, :5], new[:4, :5])   # override copies the window
+    rho = np.full((5, 6), 2000.0)
+    _, new = O.fwi_grad_update(rho, grad, 10.0, 4, 5, 0, 0, 1, 1, False, 1)
+    assert new[3, 4] == 3000.0 and new[1, 2] == 1.5   # rho bounds [1.5, 3000]
+    # full-grid loop indices address the gradient directly: cell (1, 2) with
+    # z1 = 1, x1 = 2 uses grad[1][2]; max(mat) is taken over the loop's cells
+    # only, which exclude mat[1, 1]
+    _, new = O.fwi_grad_update(mat, grad, 0.01, 4, 5, 1, 2, 1, 1, False, 2)
+    assert new[1, 2] == 2.0e9 + (0.01 * 2.0e9 / -8.0) * 2.0 and new[0, 0] == mat[0, 0]
+
+
 def test_forward_is_linear_in_source_amplitude():
     """Doubling every source doubles every output bit for bit (FP64)."""
     pb = RC.small_problem(nt=150)
