@@ -70,7 +70,7 @@ def make_plan(w, precision, window, store_mode, interval, tile, shots=None):
     plan.forward(energy=False)
     plan.set_observed([plan.records(i) for i in range(len(shots))])
     plan.set_model(lam, mu, mu_zx, rho_zp, rho_xp)
-    if tile > 0:
+    if tile != 0:
         plan.set_tile(tile)
         plan.tile = tile
     else:
@@ -237,7 +237,9 @@ def run_c2(args):
                    "gradient_store": args.store if args.store == "full" else
                    f"checkpoint every {args.interval}",
                    "fwi_time_decimation": fwi_dt,
-                   "tile": [64, plan.tile], "tile_choice": "autotuned" if args.tile <= 0 else "fixed"},
+                   "tile": [64, abs(plan.tile)],
+                   "step_kernel": "fused (one launch per step)" if plan.tile < 0 else "split (stress, velocity)",
+                   "tile_choice": "autotuned" if args.tile == 0 else "fixed"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[kname],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -31,14 +31,23 @@ struct StepParams {
     const T *lam, *mu, *mu_zx, *rho_zp, *rho_xp;
     const T* pml;  // a, b, K, a_half, b_half, K_half, each tp long
 
-    // wavefields, updated in place; per-shot stride fst
+    // wavefields; per-shot stride fst.  The split kernels update them in
+    // place; the fused kernel reads these and writes the *_w set (ping-pong).
     T *vx, *vz, *sxx, *szx, *szz;
+    T *vx_w, *vz_w, *sxx_w, *szx_w, *szz_w;
     size_t fst;
+    // Full-grid arrays (fields, model, We, ux, uz) are guard-padded: every
+    // pointer above points at cell (0, 0), which sits org elements into its
+    // slot's allocation, with at least 4 guard rows above and below the
+    // grid's tiles and 32 guard columns on the left and 4 on the right, all
+    // zero.  Halo loads up to 4 cells outside the grid stay inside the slot.
+    int org;
     T* We;         // energy weight, stride fst
     T *ux, *uz;    // adjoint displacement, stride fst
 
     // CPML memory: x-type [nzt][tp] (stride xst), z-type [tp][pitch] (stride zst)
     T *mvxx, *mvzx, *mvxz, *mvzz;      // stress side: mem_vx_x, mem_vz_x, mem_vx_z, mem_vz_z
+    T *mvxx_w, *mvzx_w, *mvxz_w, *mvzz_w;  // fused kernel: stress-side memory written here
     T *msxxx, *mszxx, *mszzz, *mszxz;  // velocity side: mem_sxx_x, mem_szx_x, mem_szz_z, mem_szx_z
     size_t xst, zst;
 
@@ -76,5 +85,22 @@ struct StepParams {
 template <typename T>
 hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s,
                        hipEvent_t e_mid = nullptr);
+
+// The explosive (sxx + szz) sources of the active shots, added after a step.
+template <typename T>
+hipError_t launch_stress_sources(const StepParams<T>& p, hipStream_t s);
+
+// Guard layout of the full-grid arrays (see StepParams::org).
+constexpr int kGuardRows = 4;
+constexpr int kGuardLeft = 32;
+constexpr int kGuardRight = 4;
+constexpr int kMaxTz = 32;
+
+// One step as a single launch (fused stress + velocity, halo ring of the
+// stresses recomputed), reading the state through vx.. and writing it
+// through vx_w..; the explosive-source launch follows when needed.  tz is 8
+// or 16.  Returns hipErrorInvalidValue for other tile heights.
+template <typename T>
+hipError_t launch_fused(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s);
 
 }  // namespace psv

@@ -531,6 +531,17 @@ __global__ void psv_stress_source_kernel(const StepParams<T> p)
     p.szz[g] = b;
 }
 
+template <typename T>
+hipError_t launch_stress_sources(const StepParams<T>& p, hipStream_t s)
+{
+    if (p.nexpl <= 0) return hipSuccess;
+    hipLaunchKernelGGL((psv_stress_source_kernel<T>), dim3((p.nexpl + 63) / 64), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+template hipError_t launch_stress_sources<float>(const StepParams<float>&, hipStream_t);
+template hipError_t launch_stress_sources<double>(const StepParams<double>&, hipStream_t);
+
 template <typename T, int ORDER, int MODE, int TZ>
 static hipError_t launch_one(const StepParams<T>& p, hipStream_t s, hipEvent_t e_mid)
 {

@@ -139,8 +139,14 @@ public:
         fpad_ = g.fdorder / 2;
         ppad_ = g.npml + fpad_;
         tp_ = 2 * (g.npml + 1);
-        pitch_ = (int)round_up((size_t)g.nxt, 64);
-        fst_ = round_up((size_t)g.nzt * pitch_, 64);
+        // guard-padded rows: kGuardLeft zero columns before cell 0, room for the
+        // last tile and kGuardRight columns after it; kGuardRows zero rows above
+        // and below the rows of the tallest tiles (psv_step.h)
+        const int tiles_x = (g.nxt + psv::kTX - 1) / psv::kTX;
+        pitch_ = (int)round_up((size_t)(psv::kGuardLeft + tiles_x * psv::kTX + psv::kGuardRight), 32);
+        rows_ = psv::kGuardRows + (int)round_up((size_t)g.nzt, psv::kMaxTz) + psv::kGuardRows;
+        fst_ = (size_t)rows_ * pitch_;
+        org_ = psv::kGuardRows * pitch_ + psv::kGuardLeft;
         xst_ = round_up((size_t)g.nzt * tp_, 64);
         zst_ = round_up((size_t)tp_ * pitch_, 64);
         pnx_ = g.nxt - 2 * ppad_ + fpad_ - 1;
@@ -187,14 +193,14 @@ public:
         const int nzt = g_.nzt, nxt = g_.nxt;
         DBuf<T>* dst[5] = {&lam_, &mu_, &mu_zx_, &rho_zp_, &rho_xp_};
         const double* src[5] = {lam, mu, mu_zx, rho_zp, rho_xp};
-        std::vector<T> h((size_t)nzt * pitch_);
+        std::vector<T> h(fst_);
         for (int a = 0; a < 5; a++) {
             const bool avg = a >= 2;  // [nzt-1][nxt-1]
             const int rows = avg ? nzt - 1 : nzt, cols = avg ? nxt - 1 : nxt;
             std::fill(h.begin(), h.end(), (T)0);
             for (int iz = 0; iz < rows; iz++)
                 for (int ix = 0; ix < cols; ix++)
-                    h[(size_t)iz * pitch_ + ix] = (T)src[a][(size_t)iz * cols + ix];
+                    h[(size_t)org_ + (size_t)iz * pitch_ + ix] = (T)src[a][(size_t)iz * cols + ix];
             PSV_HIP(dst[a]->alloc(h.size()));
             PSV_HIP(hipMemcpyAsync(dst[a]->p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
             PSV_HIP(hipStreamSynchronize(stream_));
@@ -232,7 +238,15 @@ public:
         return PSVFWI_OK;
     }
 
-    static bool tile_ok(int tz) { return tz == 4 || tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4); }
+    // tz > 0: split step kernels with 64 x tz tiles; tz < 0: the fused step
+    // kernel with 64 x |tz| tiles
+    static bool tile_ok(int tz)
+    {
+        if (tz < 0) return tz == -8 || tz == -16;
+        return tz == 4 || tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4);
+    }
+    int tz_abs() const { return tz_ < 0 ? -tz_ : tz_; }
+    bool fused() const { return tz_ < 0; }
 
     psvfwi_status set_snapshots(int interval, int field) override
     {
@@ -249,7 +263,7 @@ public:
         const int n = nsnap_slot(s);
         if (k < 0 || k >= n) return fail(PSVFWI_E_ARG, "get_snapshot: no such snapshot");
         std::vector<double> h((size_t)g_.nzt * pitch_);
-        PSV_TRY(download(snaps_.p + ((size_t)k * nslots_ + s) * fst_, h.size(), h.data()));
+        PSV_TRY(download(snaps_.p + ((size_t)k * nslots_ + s) * fst_ + org_, h.size(), h.data()));
         for (int iz = 0; iz < g_.nzt; iz++)
             std::memcpy(out + (size_t)iz * g_.nxt, h.data() + (size_t)iz * pitch_, (size_t)g_.nxt * sizeof(double));
         return PSVFWI_OK;
@@ -265,7 +279,8 @@ public:
     psvfwi_status set_tile(int tz) override
     {
         if (tz == 0) tz = kDefaultTz;
-        if (!tile_ok(tz)) return fail(PSVFWI_E_ARG, "set_tile: tile height must be 4, 8, 16 or (FP32) 32");
+        if (!tile_ok(tz))
+            return fail(PSVFWI_E_ARG, "set_tile: tile height must be 4, 8, 16 or (FP32) 32 (split step), -8 or -16 (fused step)");
         tz_ = tz;
         return PSVFWI_OK;
     }
@@ -278,15 +293,18 @@ public:
         PSV_TRY(ready());
         if (steps < 1) return fail(PSVFWI_E_ARG, "autotune: steps must be >= 1");
         PSV_TRY(alloc_state(main_));
+        PSV_TRY(alloc_state(main2_));
         PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
-        const int cand[4] = {4, 8, 16, 32};
+        const int cand[kNumCand] = {4, 8, 16, 32, -8, -16};
         float best_ms = 1e30f;
         int best_tz = tz_;
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < kNumCand; c++) {
+            tune_ms_[c] = -1;
             if (!tile_ok(cand[c])) continue;
             tz_ = cand[c];
             PSV_TRY(ensure_all_tile_lists());
             PSV_TRY(zero_state(main_));
+            PSV_TRY(zero_state(main2_));
             for (int rep = 0; rep < 2; rep++) {  // first repetition warms up
                 PSV_HIP(hipEventRecord(ev0_, stream_));
                 for (int it = 0; it < std::min(steps, ntmax_); it++) {
@@ -294,7 +312,7 @@ public:
                     p.it = it;
                     p.nslots = active(it);
                     p.flags = 0;
-                    PSV_HIP(psv::launch_step<T>(p, g_.fdorder, psv::kForward, tz_, stream_));
+                    PSV_TRY(launch_plain(p, psv::kForward, main_, main2_));
                 }
                 PSV_HIP(hipEventRecord(ev1_, stream_));
                 PSV_HIP(hipStreamSynchronize(stream_));
@@ -366,6 +384,10 @@ public:
         if (grad && !win_on_) return fail(PSVFWI_E_STATE, "forward: PSVFWI_GRADIENT needs a window");
         PSV_TRY(alloc_state(main_));
         PSV_TRY(zero_state(main_));
+        if (fused()) {
+            PSV_TRY(alloc_state(main2_));
+            PSV_TRY(zero_state(main2_));
+        }
         PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
         PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
         PSV_HIP(psv::launch_fill<T>(rec_.p, rec_.n, (T)0, stream_));
@@ -392,7 +414,7 @@ public:
             p.nslots = na;
             p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
             p.tf0 = 0;
-            PSV_TRY(launch_timed(p, psv::kForward, 0));
+            PSV_TRY(launch_timed(p, psv::kForward, 0, main_, main2_));
             if (snap && it % snap_interval_ == 0) {
                 // forward_kernel_PSV_GPU.cu:1067-1080: after the step, every snap_interval
                 const size_t k = (size_t)(it / snap_interval_);
@@ -410,6 +432,9 @@ public:
         fwd_done_ = true;
         fwd_grad_done_ = grad;
         store_loaded_ = full;
+        // with the fused kernel a shot that ended early holds its final state in
+        // the buffer set it last wrote: main_ when ntmax - nt_s is even
+        parity_ = fused();
         return PSVFWI_OK;
     }
 
@@ -515,6 +540,10 @@ public:
 
         PSV_TRY(alloc_state(main_));
         PSV_TRY(zero_state(main_));
+        if (fused()) {
+            PSV_TRY(alloc_state(main2_));
+            PSV_TRY(zero_state(main2_));
+        }
         PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
         PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
         PSV_HIP(ux_.alloc((size_t)nslots_ * fst_));
@@ -536,6 +565,10 @@ public:
             for (int it = ntmax_ - 1; it >= 0; it--) PSV_TRY(adjoint_step(it, aflags, 0));
         } else {
             PSV_TRY(alloc_state(replay_));
+            if (fused()) {
+                PSV_TRY(alloc_state(replay2_));
+                PSV_TRY(zero_state(replay2_));
+            }
             const int kk = ck_k_;
             const int nseg = (ntmax_ + kk - 1) / kk;
             const int seg_cap = (kk + w_.dt - 1) / w_.dt + 1;
@@ -552,7 +585,7 @@ public:
                     p.flags = psv::kStore;
                     p.tf0 = tf0;
                     p.We = nullptr;
-                    PSV_TRY(launch_timed(p, psv::kForward, 2));
+                    PSV_TRY(launch_timed(p, psv::kForward, 2, replay_, replay2_));
                 }
                 for (int it = t1 - 1; it >= t0; it--) PSV_TRY(adjoint_step(it, aflags, tf0));
             }
@@ -565,6 +598,7 @@ public:
         PSV_TRY(collect_samples());
         ran_ = true;
         adj_grad_done_ = grad;
+        parity_ = false;  // every shot ends at it = 0
         return PSVFWI_OK;
     }
 
@@ -584,15 +618,16 @@ public:
         if (!ran_) return fail(PSVFWI_E_STATE, "get_field: nothing has run");
         const int s = slot_of_shot_[(size_t)shot];
         const int nzt = g_.nzt, nxt = g_.nxt;
+        const State<T>& cur = (parity_ && ((ntmax_ - nt_slot_[(size_t)s]) & 1)) ? main2_ : main_;
         if (field >= 0 && field <= 7) {
             const T* base = nullptr;
-            if (field <= 4) base = main_.f[field].p;
+            if (field <= 4) base = cur.f[field].p;
             else if (field == 5) base = We_.p;
             else if (field == 6) base = ux_.p;
             else base = uz_.p;
             if (!base) return fail(PSVFWI_E_STATE, "get_field: field not allocated");
             std::vector<double> h((size_t)nzt * pitch_);
-            PSV_TRY(download(base + (size_t)s * fst_, h.size(), h.data()));
+            PSV_TRY(download(base + (size_t)s * fst_ + org_, h.size(), h.data()));
             for (int iz = 0; iz < nzt; iz++)
                 std::memcpy(out + (size_t)iz * nxt, h.data() + (size_t)iz * pitch_, (size_t)nxt * sizeof(double));
             return PSVFWI_OK;
@@ -605,12 +640,12 @@ public:
             const T* base = nullptr;
             bool xtype = k < 4;
             switch (k) {
-            case 0: base = main_.ms[0].p; break;
-            case 1: base = main_.ms[1].p; break;
+            case 0: base = cur.ms[0].p; break;
+            case 1: base = cur.ms[1].p; break;
             case 2: base = main_.mv[0].p; break;
             case 3: base = main_.mv[1].p; break;
-            case 4: base = main_.ms[2].p; break;
-            case 5: base = main_.ms[3].p; break;
+            case 4: base = cur.ms[2].p; break;
+            case 5: base = cur.ms[3].p; break;
             case 6: base = main_.mv[2].p; break;
             default: base = main_.mv[3].p; break;
             }
@@ -804,9 +839,10 @@ private:
     // for the current tile height, so a workgroup touches only its own
     psvfwi_status ensure_tile_lists(Tables& t)
     {
-        if (t.built_tz == tz_) return PSVFWI_OK;
+        const int tzh = tz_abs();
+        if (t.built_tz == tzh) return PSVFWI_OK;
         const int tiles_x = (g_.nxt + psv::kTX - 1) / psv::kTX;
-        const int tiles_z = (g_.nzt + tz_ - 1) / tz_;
+        const int tiles_z = (g_.nzt + tzh - 1) / tzh;
         const size_t ntile = (size_t)tiles_x * tiles_z;
         auto build = [&](const std::vector<int>& soff, const std::vector<int>& zx, DBuf<int>& doff,
                          DBuf<int>& didx) -> psvfwi_status {
@@ -815,7 +851,7 @@ private:
             for (int s = 0; s < nslots_; s++)
                 for (int k = soff[(size_t)s]; k < soff[(size_t)s + 1]; k++) {
                     const int z = zx[2 * (size_t)k], x = zx[2 * (size_t)k + 1];
-                    const size_t tix = (size_t)s * ntile + (size_t)(z / tz_) * tiles_x + (size_t)(x / psv::kTX);
+                    const size_t tix = (size_t)s * ntile + (size_t)(z / tzh) * tiles_x + (size_t)(x / psv::kTX);
                     items.emplace_back(tix, k);
                 }
             std::stable_sort(items.begin(), items.end(),
@@ -832,7 +868,7 @@ private:
         };
         PSV_TRY(build(t.grp_off, t.grp_zx, t.tinj_off, t.tinj_idx));
         PSV_TRY(build(t.rec_off, t.rec_zx, t.trec_off, t.trec_idx));
-        t.built_tz = tz_;
+        t.built_tz = tzh;
         return PSVFWI_OK;
     }
 
@@ -979,7 +1015,7 @@ private:
         p.nslots = active(it);
         p.flags = flags;
         p.tf0 = tf0;
-        PSV_TRY(launch_timed(p, psv::kAdjoint, 1));
+        PSV_TRY(launch_timed(p, psv::kAdjoint, 1, main_, main2_));
         return PSVFWI_OK;
     }
 
@@ -989,17 +1025,50 @@ private:
     static constexpr int kSampleEvery = 16;
     static constexpr int kMaxSamples = 512;
 
-    psvfwi_status launch_timed(const psv::StepParams<T>& p, int mode, int kind)
+    // one step on st: the split kernels update st in place; the fused kernel
+    // reads st, writes alt, and the two swap roles (st is current after it)
+    psvfwi_status launch_plain(psv::StepParams<T> p, int mode, State<T>& st, State<T>& alt,
+                               hipEvent_t e_mid = nullptr)
+    {
+        if (!fused()) {
+            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, mode, tz_, stream_, e_mid));
+            return PSVFWI_OK;
+        }
+        p.vx_w = at(alt.f[kF_VX]);
+        p.vz_w = at(alt.f[kF_VZ]);
+        p.sxx_w = at(alt.f[kF_SXX]);
+        p.szx_w = at(alt.f[kF_SZX]);
+        p.szz_w = at(alt.f[kF_SZZ]);
+        p.mvxx_w = alt.ms[0].p;
+        p.mvzx_w = alt.ms[1].p;
+        p.mvxz_w = alt.ms[2].p;
+        p.mvzz_w = alt.ms[3].p;
+        PSV_HIP(psv::launch_fused<T>(p, g_.fdorder, mode, tz_abs(), stream_));
+        if (e_mid) PSV_HIP(hipEventRecord(e_mid, stream_));
+        for (int f = 0; f < 5; f++) swap_buf(st.f[f], alt.f[f]);
+        for (int m = 0; m < 4; m++) swap_buf(st.ms[m], alt.ms[m]);
+        return PSVFWI_OK;
+    }
+
+    static void swap_buf(DBuf<T>& a, DBuf<T>& b)
+    {
+        std::swap(a.p, b.p);
+        std::swap(a.n, b.n);
+    }
+
+    T* at(DBuf<T>& b) const { return b.p ? b.p + org_ : nullptr; }
+
+    psvfwi_status launch_timed(const psv::StepParams<T>& p, int mode, int kind, State<T>& st, State<T>& alt)
     {
         const bool sample = sample_ev_[0][0] && (launches_ % kSampleEvery == 0) && nsamp_ < kMaxSamples;
         if (sample) {
             PSV_HIP(hipEventRecord(sample_ev_[nsamp_][0], stream_));
-            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, mode, tz_, stream_, sample_ev_[nsamp_][1]));
+            PSV_TRY(launch_plain(p, mode, st, alt, sample_ev_[nsamp_][1]));
             PSV_HIP(hipEventRecord(sample_ev_[nsamp_][2], stream_));
             samp_kind_[nsamp_] = kind;
             nsamp_++;
         } else {
-            PSV_HIP(psv::launch_step<T>(p, g_.fdorder, mode, tz_, stream_));
+            PSV_TRY(launch_plain(p, mode, st, alt));
         }
         launches_++;
         return PSVFWI_OK;
@@ -1049,32 +1118,42 @@ private:
         p.pnx = pnx_;
         p.pnz = pnz_;
         p.tiles_x = (g_.nxt + psv::kTX - 1) / psv::kTX;
-        p.tiles_z = (g_.nzt + tz_ - 1) / tz_;
+        p.tiles_z = (g_.nzt + tz_abs() - 1) / tz_abs();
+        p.org = org_;
         p.it_dev = nullptr;
         p.dt = (T)g_.dt;
         p.dxi = (T)1.0 / (T)g_.dx;
         p.dzi = (T)1.0 / (T)g_.dz;
         p.hc1 = (T)g_.hc[1];
         p.hc2 = (T)g_.hc[2];
-        p.lam = lam_.p;
-        p.mu = mu_.p;
-        p.mu_zx = mu_zx_.p;
-        p.rho_zp = rho_zp_.p;
-        p.rho_xp = rho_xp_.p;
+        p.lam = at(lam_);
+        p.mu = at(mu_);
+        p.mu_zx = at(mu_zx_);
+        p.rho_zp = at(rho_zp_);
+        p.rho_xp = at(rho_xp_);
         p.pml = pml_.p;
-        p.vx = st.f[kF_VX].p;
-        p.vz = st.f[kF_VZ].p;
-        p.sxx = st.f[kF_SXX].p;
-        p.szx = st.f[kF_SZX].p;
-        p.szz = st.f[kF_SZZ].p;
+        p.vx = at(st.f[kF_VX]);
+        p.vz = at(st.f[kF_VZ]);
+        p.sxx = at(st.f[kF_SXX]);
+        p.szx = at(st.f[kF_SZX]);
+        p.szz = at(st.f[kF_SZZ]);
+        p.vx_w = p.vx;
+        p.vz_w = p.vz;
+        p.sxx_w = p.sxx;
+        p.szx_w = p.szx;
+        p.szz_w = p.szz;
         p.fst = fst_;
-        p.We = We_.p;
-        p.ux = ux_.p;
-        p.uz = uz_.p;
+        p.We = at(We_);
+        p.ux = at(ux_);
+        p.uz = at(uz_);
         p.mvxx = st.ms[0].p;
         p.mvzx = st.ms[1].p;
         p.mvxz = st.ms[2].p;
         p.mvzz = st.ms[3].p;
+        p.mvxx_w = p.mvxx;
+        p.mvzx_w = p.mvzx;
+        p.mvxz_w = p.mvxz;
+        p.mvzz_w = p.mvzz;
         p.msxxx = st.mv[0].p;
         p.mszxx = st.mv[1].p;
         p.mszzz = st.mv[2].p;
@@ -1122,7 +1201,8 @@ private:
     }
 
     psvfwi_grid_desc g_;
-    int fpad_ = 1, ppad_ = 0, tp_ = 2, pitch_ = 0, pnx_ = 0, pnz_ = 0, tz_ = 16;
+    int fpad_ = 1, ppad_ = 0, tp_ = 2, pitch_ = 0, rows_ = 0, org_ = 0, pnx_ = 0, pnz_ = 0, tz_ = 16;
+    bool parity_ = false;
     size_t fst_ = 0, xst_ = 0, zst_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -1140,6 +1220,7 @@ private:
     int nslots_ = 0, ntmax_ = 0;
 
     State<T> main_, replay_;
+    State<T> main2_, replay2_;  // partners of the fused (ping-pong) step
     DBuf<T> We_, ux_, uz_, store_, grad_, rec_, adj_sig_, ckpt_;
     size_t sst_ = 0, gst_ = 0;
     Tables fwd_tab_, adj_tab_, fwd_tab_noreceivers_;
@@ -1161,7 +1242,8 @@ private:
     bool have_obs_ = false;
     double last_ms_ = 0;
     long launches_ = 0;
-    float tune_ms_[4] = {0, 0, 0, 0};
+    static constexpr int kNumCand = 6;
+    float tune_ms_[kNumCand] = {0, 0, 0, 0, 0, 0};
     hipEvent_t sample_ev_[kMaxSamples][3] = {};
     int samp_kind_[kMaxSamples] = {};
     int nsamp_ = 0;

@@ -108,11 +108,16 @@ psvfwi_status psvfwi_set_store_mode(psvfwi_plan* plan, int mode, int interval);
 /* Replaces the batch of shots.  Shot i keeps index i in every getter. */
 psvfwi_status psvfwi_set_shots(psvfwi_plan* plan, int nshots, const psvfwi_shot_desc* shots);
 
-/* Tile height of the step kernel (4, 8, 16 or 32; FP64 4, 8 or 16); 0 = default (8). */
+/* Step kernel and tile height.  tz > 0: two launches per step (stress, then
+ * velocity, in place) on 64 x tz tiles, tz = 4, 8, 16 or 32 (FP64: up to 16);
+ * tz < 0: one fused launch per step on 64 x |tz| tiles, tz = -8 or -16
+ * (ping-pong state, stress ring recomputed); 0 = default (8).  Every choice
+ * gives the same results bit for bit. */
 psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
 
 /* Autotuner: times `steps` forward steps of the current batch with every
- * tile height, keeps the fastest and returns it in *best_tile (may be NULL).
+ * step kernel and tile height of psvfwi_set_tile, keeps the fastest and
+ * returns its psvfwi_set_tile value in *best_tile (may be NULL).
  * Needs set_model / set_pml / set_shots first.  (No counterpart in the
  * reference, whose launch shape is fixed at 16 x 16,
  * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */

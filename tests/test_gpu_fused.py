@@ -1,0 +1,150 @@
+"""The fused step kernel (one launch per time step, ping-pong state,
+set with a negative tile height) against the oracle, bit for bit, on the
+same cases as the split kernels: orders 2/4 with and without free surface
+(wavefields and CPML memory), gradients with full and decimated windows,
+checkpointed gradients, uneven batches (final state of shots that end
+early), explosive / vx sources, boundary geometry and snapshots."""
+import numpy as np
+import pytest
+
+import psv_oracle as O
+import refcase as RC
+from gpucase import plan_of, shots_of
+from test_gpu_parity import CASES, FIELDS, MEMS, WINDOWS, _fwi_problem, _oracle_gradient, _oracle_mems
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"o{c['fdorder']}fs{c['fsurf']}")
+@pytest.mark.parametrize("prec,tz", [("fp64", -8), ("fp32", -8), ("fp32", -16), ("fp64", -16)])
+def test_fused_forward_bit_exact(case, prec, tz):
+    print(f"fused forward {case} {prec} {tz}", flush=True)
+    pb = RC.small_problem(nx=90, nz=50, npml=8, nt=260, **case)
+    ref = O.forward(pb, prec="f64" if prec == "fp64" else "f32")
+    plan = plan_of(pb, prec, tz=tz)
+    plan.forward(energy=True)
+    assert np.array_equal(plan.records(), ref["rec"].astype(np.float64))
+    for k in FIELDS:
+        assert np.array_equal(plan.field(k), ref[k].astype(np.float64)), k
+    om = _oracle_mems(pb, ref["mem"])
+    for k in MEMS:
+        assert np.array_equal(plan.field(k), om[k].astype(np.float64)), k
+
+
+@pytest.mark.parametrize("win", list(WINDOWS))
+@pytest.mark.parametrize("fdorder", [2, 4])
+def test_fused_gradient_fp64_bit_exact(fdorder, win):
+    print(f"fused gradient fp64 order {fdorder} window {win}", flush=True)
+    pb, obs = _fwi_problem(fdorder=fdorder, fwi=WINDOWS[win])
+    f, m, res, a = _oracle_gradient(pb, obs)
+    plan = plan_of(pb, "fp64", tz=-16)
+    plan.forward(energy=True, gradient=True)
+    assert np.array_equal(plan.store(), f["store"])
+    plan.misfit([obs])
+    plan.adjoint(energy=True, gradient=True)
+    for got, k in zip(plan.gradient(0), ("grad_lam", "grad_mu", "grad_rho")):
+        assert np.array_equal(got, a[k]), k
+    for k in FIELDS:
+        assert np.array_equal(plan.field(k), a[k]), k
+
+
+def test_fused_checkpoint_gradient_equals_full_store():
+    print("fused checkpoint vs store", flush=True)
+    pb, obs = _fwi_problem(nt=333, fwi=WINDOWS["decimated"])
+    plan = plan_of(pb, "fp32", tz=-8)
+    plan.forward(gradient=True)
+    plan.misfit([obs])
+    plan.adjoint(gradient=True)
+    g_full = plan.gradient(0)
+    split = plan_of(pb, "fp32", tz=8)
+    split.forward(gradient=True)
+    split.misfit([obs])
+    split.adjoint(gradient=True)
+    for a, b in zip(g_full, split.gradient(0)):
+        assert np.array_equal(a, b)
+    for k in (7, 50, 333):
+        plan.set_store_mode("checkpoint", k)
+        plan.forward(gradient=True)
+        plan.misfit([obs])
+        plan.adjoint(gradient=True)
+        for a, b in zip(g_full, plan.gradient(0)):
+            assert np.array_equal(a, b), k
+
+
+def test_fused_batched_uneven_shots():
+    """Shots that end before the longest one keep their own final state
+    (which of the two buffer sets holds it depends on nt)."""
+    print("fused batched uneven shots", flush=True)
+    base = RC.small_problem(nx=80, nz=48, npml=8, nt=10, fdorder=4, fsurf=1, hetero=True)
+    rng = np.random.default_rng(9)
+    shots, obs = [], []
+    for i, nt in enumerate((150, 231, 90, 230)):
+        nrec = int(rng.integers(3, 12))
+        xs = int(rng.integers(base.ppad + 5, base.nxt - base.ppad - 5))
+        sig = O.create_wavelet(nt, base.dt, 1.0 + i, 25.0, 0.0)[None]
+        rec = dict(x=list(np.linspace(base.ppad, base.nxt - base.ppad - 1, nrec).astype(int)),
+                   z=[base.fpad + 1] * nrec, comp=[2] * nrec)
+        shots.append(dict(nt=nt, src=dict(x=[xs], z=[base.fpad + 3], comp=[2]), src_signal=sig,
+                          rec=rec))
+        obs.append(rng.standard_normal((nrec, nt)) * 1e-3)
+    win = RC.full_window(base, dt=2)
+    batch = plan_of(base, "fp32", shots=shots, tz=-16)
+    batch.set_window(win)
+    batch.forward(gradient=True)
+    fwd_fields = [{k: batch.field(k, i) for k in FIELDS + MEMS} for i in range(len(shots))]
+    mb = batch.misfit(obs)
+    batch.adjoint(gradient=True)
+    for i, s in enumerate(shots):
+        one = plan_of(base, "fp32", shots=[s], tz=8)
+        one.set_window(win)
+        one.forward(gradient=True)
+        assert np.array_equal(one.records(), batch.records(i))
+        for k in FIELDS + MEMS:
+            assert np.array_equal(one.field(k), fwd_fields[i][k]), (i, k)
+        assert one.misfit([obs[i]])[0] == mb[i]
+        one.adjoint(gradient=True)
+        for a, b in zip(one.gradient(0), batch.gradient(i)):
+            assert np.array_equal(a, b)
+        for k in FIELDS:
+            assert np.array_equal(one.field(k), batch.field(k, i)), (i, k)
+
+
+def test_fused_explosive_vx_sources_and_boundary_geometry():
+    print("fused explosive / vx sources, boundary geometry", flush=True)
+    pb = RC.small_problem(nx=60, nz=60, npml=8, nt=200, fdorder=4, fsurf=0, nsrc=3, src_comp=4)
+    pb.src = dict(pb.src)
+    pb.src["comp"] = [4, 1, 2]
+    pb.src["x"] = [pb.src["x"][0], pb.src["x"][0], pb.src["x"][2]]
+    pb.src["z"] = list(pb.src["z"])
+    ref = O.forward(pb)
+    plan = plan_of(pb, "fp64", tz=-8)
+    plan.forward()
+    assert np.array_equal(plan.records(), ref["rec"])
+    for k in FIELDS:
+        assert np.array_equal(plan.field(k), ref[k])
+    pb = RC.small_problem(nx=40, nz=30, npml=0, nt=120, fdorder=2, fsurf=1, nsrc=2)
+    pb.src = dict(x=[20, 20, 0], z=[5, 5, 0], comp=[2, 2, 2], shot=[0, 0, 0])
+    pb.src_signal = np.vstack([pb.src_signal, pb.src_signal[:1] * 0.5])
+    pb.rec = dict(x=[0, 15, 41], z=[0, 0, 7], comp=[2, 1, 2], shot=[0, 0, 0])
+    ref = O.forward(pb)
+    plan = plan_of(pb, "fp64", tz=-16)
+    plan.forward()
+    assert np.array_equal(plan.records(), ref["rec"])
+    for k in FIELDS:
+        assert np.array_equal(plan.field(k), ref[k])
+
+
+def test_fused_snapshots_and_autotune():
+    print("fused snapshots, autotune", flush=True)
+    pb = RC.small_problem(nx=70, nz=40, npml=8, nt=121, fdorder=4, fsurf=1)
+    plan = plan_of(pb, "fp64", tz=-16)
+    plan.set_snapshots(40, "szz")
+    plan.forward()
+    snaps = plan.snapshots(0)
+    ref = O.forward(RC.small_problem(nx=70, nz=40, npml=8, nt=81, fdorder=4, fsurf=1))
+    assert np.array_equal(snaps[2], ref["szz"])
+    p32 = plan_of(pb, "fp32")
+    best = p32.autotune(20)
+    assert best in (4, 8, 16, 32, -8, -16)
+    p32.forward()
+    assert np.array_equal(p32.records(), O.forward(pb, prec="f32")["rec"].astype(np.float64))
