@@ -19,10 +19,13 @@
 // operation order; the library is built with -ffp-contract=off, so the FP64
 // instantiation reproduces the FP64 CPU path bit for bit.
 //
-// A workgroup is 256 threads (4 waves) on a 64 x TZ tile; the neighbour
-// values a tile needs (a 2-cell halo of the other field group) are staged in
-// LDS row by row (64 lanes + 4 halo lanes per row), everything else is one
-// coalesced load per cell.
+// A workgroup is 256 threads (4 waves) on a 64 x TZ tile.  The neighbour
+// values a tile needs (the other field group within 2 cells) are staged in
+// LDS in the shape of the stencils: full rows of the 64 tile columns above
+// and below the tile, and the 4 halo columns only on the tile's own rows (one
+// pass of the first wave).  Everything else is one coalesced load per cell.
+// The arrays are guard-padded (StepParams::org), so loads are not predicated;
+// offsets are 32-bit byte offsets from uniform per-slot bases.
 #include <hip/hip_runtime.h>
 
 #include "psv_step.h"
@@ -60,6 +63,18 @@ __device__ __forceinline__ TileId tile_of(const StepParams<T>& p)
     return t;
 }
 
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, unsigned boff)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+
+template <typename T>
+__device__ __forceinline__ void stg(T* base, unsigned boff, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
 // ---------------------------------------------------------------- stress
 template <typename T, int ORDER, int MODE, int TZ>
 __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T> p)
@@ -67,76 +82,92 @@ __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T
     constexpr int H = ORDER / 2;                 // fpad
     constexpr int VW = kTX + 2 * kHalo, VHT = TZ + 2 * kHalo;
     constexpr int NR = TZ / kRowsPerWave;
+    constexpr int NH = (VHT + kRowsPerWave - 1) / kRowsPerWave;
+    constexpr int NHC = 2 * kHalo * TZ;          // halo-column cells (own rows only)
+    static_assert(NHC <= kThreads, "halo columns in one pass");
+    // vx and vz: the x derivatives need columns -2..+2 on the own rows, the
+    // z derivatives rows -2..+2 on the own columns; the corners are not used
     __shared__ T lvx[VHT][VW];
     __shared__ T lvz[VHT][VW];
 
     const TileId t = tile_of<TZ>(p);
     const int slot = t.slot, z0 = t.z0, x0 = t.x0;
     const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;
-    const int nzt = p.nzt, nxt = p.nxt, pitch = p.pitch;
+    const int nt_s = p.shot_nt[slot];
+    if (it >= nt_s) return;  // this shot has finished (uniform over the block)
+    const int nzt = p.nzt, nxt = p.nxt, pitch = p.pitch, org = p.org;
     const int fpad = H, ppad = p.ppad;
-    const int tx = threadIdx.x & (kTX - 1), ty = threadIdx.x / kTX;
-    const size_t fo = (size_t)slot * p.fst;
+    const int tid = threadIdx.x;
+    const int tx = tid & (kTX - 1), ty = tid / kTX;
     const bool fwi_step = (p.flags & kStore) && (it % p.fwi_dt == 0);
 
+    // uniform bases at the start of the slot's allocation (guard-padded
+    // layout): every load below stays inside it, so none is predicated
+    const ptrdiff_t so = (ptrdiff_t)slot * (ptrdiff_t)p.fst - org;
+    const T* vxA = p.vx + so;
+    const T* vzA = p.vz + so;
+    T* sxxS = p.sxx + so;
+    T* szxS = p.szx + so;
+    T* szzS = p.szz + so;
+    T* uxS = p.ux + so;
+    T* uzS = p.uz + so;
+    const T* lamM = p.lam - org;
+    const T* muM = p.mu - org;
+    const T* mzxM = p.mu_zx - org;
+    auto boff = [&](int gz, int gx) -> unsigned { return (unsigned)((org + gz * pitch + gx) * (int)sizeof(T)); };
+
     // ---- loads: own cells (stresses, model, adjoint displacement) and the
-    // velocity tile with its halo, all issued before any is used
+    // velocity stencil region, all issued before any is used
     T rsxx[NR], rszx[NR], rszz[NR], rlam[NR], rmu[NR], rmzx[NR], rux[NR], ruz[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
-        const int iz = z0 + ty + k * kRowsPerWave, ix = x0 + tx;
-        rsxx[k] = rszx[k] = rszz[k] = rlam[k] = rmu[k] = rmzx[k] = rux[k] = ruz[k] = 0;
-        if (iz < nzt && ix < nxt) {
-            const size_t g = (size_t)iz * pitch + ix;
-            rsxx[k] = p.sxx[fo + g];
-            rszx[k] = p.szx[fo + g];
-            rszz[k] = p.szz[fo + g];
-            rlam[k] = p.lam[g];
-            rmu[k] = p.mu[g];
-            rmzx[k] = p.mu_zx[g];
-            if (MODE == kAdjoint) {
-                rux[k] = p.ux[fo + g];
-                ruz[k] = p.uz[fo + g];
-            }
+        const unsigned o = boff(z0 + ty + k * kRowsPerWave, x0 + tx);
+        rsxx[k] = ldg((const T*)sxxS, o);
+        rszx[k] = ldg((const T*)szxS, o);
+        rszz[k] = ldg((const T*)szzS, o);
+        rlam[k] = ldg(lamM, o);
+        rmu[k] = ldg(muM, o);
+        rmzx[k] = ldg(mzxM, o);
+        rux[k] = ruz[k] = 0;
+        if (MODE == kAdjoint) {
+            rux[k] = ldg((const T*)uxS, o);
+            ruz[k] = ldg((const T*)uzS, o);
         }
     }
-    const int nt_s = p.shot_nt[slot];
-    // velocity tile: row r = ty + 4k, lanes cover columns 0..63 and lanes
-    // 0..3 the four extra halo columns; every load is issued before the
-    // first LDS store (constant trip counts, fully unrolled)
-    constexpr int NH = (VHT + kRowsPerWave - 1) / kRowsPerWave;
-    T hvx[NH][2], hvz[NH][2];
+    T hvx[NH], hvz[NH];
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         const int r = ty + k * kRowsPerWave;
-        const int gz = z0 - kHalo + r;
-        const bool zin = r < VHT && gz >= 0 && gz < nzt;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int gx = x0 - kHalo + (h ? kTX : 0) + tx;
-            hvx[k][h] = 0;
-            hvz[k][h] = 0;
-            if (zin && (h == 0 || tx < 2 * kHalo) && gx >= 0 && gx < nxt) {
-                const size_t g = fo + (size_t)gz * pitch + gx;
-                hvx[k][h] = p.vx[g];
-                hvz[k][h] = p.vz[g];
-            }
+        hvx[k] = hvz[k] = 0;
+        if (r < VHT) {
+            const unsigned o = boff(z0 - kHalo + r, x0 + tx);
+            hvx[k] = ldg(vxA, o);
+            hvz[k] = ldg(vzA, o);
         }
+    }
+    T cvx = 0, cvz = 0;
+    int cr = 0, cc = 0;
+    if (tid < NHC) {
+        cr = kHalo + tid / (2 * kHalo);
+        const int c = tid % (2 * kHalo);
+        cc = c < kHalo ? c : kTX + c;
+        const unsigned o = boff(z0 - kHalo + cr, x0 - kHalo + cc);
+        cvx = ldg(vxA, o);
+        cvz = ldg(vzA, o);
     }
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         const int r = ty + k * kRowsPerWave;
         if (r < VHT) {
-            lvx[r][tx] = hvx[k][0];
-            lvz[r][tx] = hvz[k][0];
-            if (tx < 2 * kHalo) {
-                lvx[r][kTX + tx] = hvx[k][1];
-                lvz[r][kTX + tx] = hvz[k][1];
-            }
+            lvx[r][kHalo + tx] = hvx[k];
+            lvz[r][kHalo + tx] = hvz[k];
         }
     }
+    if (tid < NHC) {
+        lvx[cr][cc] = cvx;
+        lvz[cr][cc] = cvz;
+    }
     __syncthreads();
-    if (it >= nt_s) return;  // this shot has finished (uniform over the block)
 
     const T dt = p.dt, dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
     const int tp = p.tp, npml = p.npml;
@@ -156,7 +187,7 @@ __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T
         const int rr = ty + k * kRowsPerWave;
         const int iz = z0 + rr, ix = x0 + tx;
         if (iz >= nzt || ix >= nxt) continue;
-        const size_t g = (size_t)iz * pitch + ix;
+        const unsigned o = boff(iz, ix);
         const int vr = rr + kHalo, vc = tx + kHalo;
 
         // -- work on the old state: forward store / adjoint correlation
@@ -168,8 +199,8 @@ __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T
                 if (iz >= fpad && iz < nzt - fpad && ix >= fpad && ix < nxt - fpad) {
                     uzn = uzn + dt * lvz[vr][vc];
                     uxn = uxn + dt * lvx[vr][vc];
-                    p.uz[fo + g] = uzn;
-                    p.ux[fo + g] = uxn;
+                    stg(uzS, o, uzn);
+                    stg(uxS, o, uxn);
                 }
             }
             const int dzw = iz - p.fwi_z1, dxw = ix - p.fwi_x1;
@@ -269,9 +300,9 @@ __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T
             szxn = (T)0.0;
             sxxn = (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
         }
-        p.sxx[fo + g] = sxxn;
-        p.szx[fo + g] = szxn;
-        p.szz[fo + g] = szzn;
+        stg(sxxS, o, sxxn);
+        stg(szxS, o, szxn);
+        stg(szzS, o, szzn);
     }
 }
 
@@ -282,9 +313,14 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
     constexpr int H = ORDER / 2;
     constexpr int SW = kTX + 2 * kHalo, SHT = TZ + 2 * kHalo;
     constexpr int NR = TZ / kRowsPerWave;
-    __shared__ T lsxx[SHT][SW];
+    constexpr int NH = (SHT + kRowsPerWave - 1) / kRowsPerWave;
+    constexpr int NHC = 2 * kHalo * TZ;          // halo-column cells (own rows only)
+    static_assert(NHC <= kThreads, "halo columns in one pass");
+    // the stencil each stress needs: sxx columns -1..+2 on the own rows,
+    // szz rows -1..+2 on the own columns, szx both (no corners)
+    __shared__ T lsxx[TZ][SW];
     __shared__ T lszx[SHT][SW];
-    __shared__ T lszz[SHT][SW];
+    __shared__ T lszz[SHT][kTX];
     __shared__ T lvx[TZ][kTX];
     __shared__ T lvz[TZ][kTX];
     __shared__ int lmeta[4];
@@ -292,30 +328,40 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
     const TileId t = tile_of<TZ>(p);
     const int slot = t.slot, z0 = t.z0, x0 = t.x0;
     const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;
-    const int nzt = p.nzt, nxt = p.nxt, pitch = p.pitch;
+    const int nt_s = p.shot_nt[slot];
+    if (it >= nt_s) return;  // uniform over the block
+    const int nzt = p.nzt, nxt = p.nxt, pitch = p.pitch, org = p.org;
     const int fpad = H, ppad = p.ppad;
     const int tid = threadIdx.x;
     const int tx = tid & (kTX - 1), ty = tid / kTX;
-    const size_t fo = (size_t)slot * p.fst;
     const bool fsurf = p.fsurf != 0;
     const bool energy = (p.flags & kEnergy) != 0;
+    // rows above the free surface are staged only by the top tiles
+    const bool top = fsurf && z0 - kHalo < fpad;
 
-    // ---- loads: own cells and the stress tile with its halo
-    T rvx[NR], rvz[NR], rrxp[NR], rrzp[NR], rwe[NR];
+    const ptrdiff_t so = (ptrdiff_t)slot * (ptrdiff_t)p.fst - org;
+    const T* sxxA = p.sxx + so;
+    T* szxS = p.szx + so;
+    T* szzS = p.szz + so;
+    T* vxS = p.vx + so;
+    T* vzS = p.vz + so;
+    T* WeS = p.We + so;
+    const T* rxpM = p.rho_xp - org;
+    const T* rzpM = p.rho_zp - org;
+    auto boff = [&](int gz, int gx) -> unsigned { return (unsigned)((org + gz * pitch + gx) * (int)sizeof(T)); };
+
+    // ---- loads: own cells and the stress stencil region
+    T rvx[NR], rvz[NR], rrxp[NR], rrzp[NR], rwe[NR], hsxx[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
-        const int iz = z0 + ty + k * kRowsPerWave, ix = x0 + tx;
-        rvx[k] = rvz[k] = rrxp[k] = rrzp[k] = rwe[k] = 0;
-        if (iz < nzt && ix < nxt) {
-            const size_t g = (size_t)iz * pitch + ix;
-            rvx[k] = p.vx[fo + g];
-            rvz[k] = p.vz[fo + g];
-            rrxp[k] = p.rho_xp[g];
-            rrzp[k] = p.rho_zp[g];
-            if (energy) rwe[k] = p.We[fo + g];
-        }
+        const unsigned o = boff(z0 + ty + k * kRowsPerWave, x0 + tx);
+        rvx[k] = ldg((const T*)vxS, o);
+        rvz[k] = ldg((const T*)vzS, o);
+        rrxp[k] = ldg(rxpM, o);
+        rrzp[k] = ldg(rzpM, o);
+        rwe[k] = energy ? ldg((const T*)WeS, o) : (T)0;
+        hsxx[k] = ldg(sxxA, o);
     }
-    // this tile's source / receiver list bounds, by vector loads
     int meta = 0;
     {
         const size_t tix = (size_t)slot * p.tiles_x * p.tiles_z + t.tile;
@@ -324,50 +370,54 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
         else if (tid == 2) meta = p.trec_off[tix];
         else if (tid == 3) meta = p.trec_off[tix + 1];
     }
-    const int nt_s = p.shot_nt[slot];
-    // stress tile, loaded like the velocity tile of the stress kernel; rows
-    // above the free surface hold the mirror of the new stresses below it
+    // szx, szz on the own columns, rows -2..TZ+1; above the free surface the
+    // rows hold the mirror of the new stresses below it
     // (forward_kernel_PSV_GPU.cu:935-942), so they are read there
-    constexpr int NH = (SHT + kRowsPerWave - 1) / kRowsPerWave;
-    T hsxx[NH][2], hszx[NH][2], hszz[NH][2];
+    T hszx[NH], hszz[NH];
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         const int r = ty + k * kRowsPerWave;
-        const int gz = z0 - kHalo + r;
-        const bool ghost = fsurf && gz >= 0 && gz < fpad;
-        const int mz = ghost ? 2 * fpad - gz : gz;
-        const bool zin = r < SHT && gz >= 0 && gz < nzt;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int gx = x0 - kHalo + (h ? kTX : 0) + tx;
-            hsxx[k][h] = hszx[k][h] = hszz[k][h] = 0;
-            if (zin && (h == 0 || tx < 2 * kHalo) && gx >= 0 && gx < nxt) {
-                const size_t g = fo + (size_t)gz * pitch + gx;
-                const bool mirror = ghost && gx >= fpad && gx < nxt - fpad;
-                const size_t gm = mirror ? fo + (size_t)mz * pitch + gx : g;
-                hsxx[k][h] = p.sxx[g];
-                hszx[k][h] = mirror ? -p.szx[gm] : p.szx[gm];
-                hszz[k][h] = mirror ? -p.szz[gm] : p.szz[gm];
+        const int gz = z0 - kHalo + r, gx = x0 + tx;
+        hszx[k] = hszz[k] = 0;
+        if (r < SHT) {
+            if (top && gz >= 0 && gz < fpad && gx >= fpad && gx < nxt - fpad) {
+                const unsigned om = boff(2 * fpad - gz, gx);
+                hszx[k] = -ldg((const T*)szxS, om);
+                hszz[k] = -ldg((const T*)szzS, om);
+            } else {
+                const unsigned o = boff(gz, gx);
+                hszx[k] = ldg((const T*)szxS, o);
+                hszz[k] = ldg((const T*)szzS, o);
             }
         }
     }
+    // sxx and szx on the halo columns of the own rows
+    T csxx = 0, cszx = 0;
+    int cj = 0, cc = 0;
+    if (tid < NHC) {
+        cj = tid / (2 * kHalo);
+        const int c = tid % (2 * kHalo);
+        cc = c < kHalo ? c : kTX + c;
+        const unsigned o = boff(z0 + cj, x0 - kHalo + cc);
+        csxx = ldg(sxxA, o);
+        cszx = ldg((const T*)szxS, o);
+    }
+#pragma unroll
+    for (int k = 0; k < NR; k++) lsxx[ty + k * kRowsPerWave][kHalo + tx] = hsxx[k];
 #pragma unroll
     for (int k = 0; k < NH; k++) {
         const int r = ty + k * kRowsPerWave;
         if (r < SHT) {
-            lsxx[r][tx] = hsxx[k][0];
-            lszx[r][tx] = hszx[k][0];
-            lszz[r][tx] = hszz[k][0];
-            if (tx < 2 * kHalo) {
-                lsxx[r][kTX + tx] = hsxx[k][1];
-                lszx[r][kTX + tx] = hszx[k][1];
-                lszz[r][kTX + tx] = hszz[k][1];
-            }
+            lszx[r][kHalo + tx] = hszx[k];
+            lszz[r][tx] = hszz[k];
         }
+    }
+    if (tid < NHC) {
+        lsxx[cj][cc] = csxx;
+        lszx[kHalo + cj][cc] = cszx;
     }
     if (tid < 4) lmeta[tid] = meta;
     __syncthreads();
-    if (it >= nt_s) return;  // uniform over the block
 
     const T dt = p.dt, dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
     const int tp = p.tp, npml = p.npml;
@@ -390,15 +440,15 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
         const int sr = rr + kHalo, sc = tx + kHalo;
         T sxx_x, szx_x, szx_z, szz_z;
         if (ORDER == 2) {
-            sxx_x = dxi * hc1 * (lsxx[sr][sc + 1] - lsxx[sr][sc]);
+            sxx_x = dxi * hc1 * (lsxx[rr][sc + 1] - lsxx[rr][sc]);
             szx_x = dxi * hc1 * (lszx[sr][sc] - lszx[sr][sc - 1]);
             szx_z = dzi * hc1 * (lszx[sr][sc] - lszx[sr - 1][sc]);
-            szz_z = dzi * hc1 * (lszz[sr + 1][sc] - lszz[sr][sc]);
+            szz_z = dzi * hc1 * (lszz[sr + 1][tx] - lszz[sr][tx]);
         } else {
-            sxx_x = dxi * (hc1 * (lsxx[sr][sc + 1] - lsxx[sr][sc]) + hc2 * (lsxx[sr][sc + 2] - lsxx[sr][sc - 1]));
+            sxx_x = dxi * (hc1 * (lsxx[rr][sc + 1] - lsxx[rr][sc]) + hc2 * (lsxx[rr][sc + 2] - lsxx[rr][sc - 1]));
             szx_x = dxi * (hc1 * (lszx[sr][sc] - lszx[sr][sc - 1]) + hc2 * (lszx[sr][sc + 1] - lszx[sr][sc - 2]));
             szx_z = dzi * (hc1 * (lszx[sr][sc] - lszx[sr - 1][sc]) + hc2 * (lszx[sr + 1][sc] - lszx[sr - 2][sc]));
-            szz_z = dzi * (hc1 * (lszz[sr + 1][sc] - lszz[sr][sc]) + hc2 * (lszz[sr + 2][sc] - lszz[sr - 1][sc]));
+            szz_z = dzi * (hc1 * (lszz[sr + 1][tx] - lszz[sr][tx]) + hc2 * (lszz[sr + 2][tx] - lszz[sr - 1][tx]));
         }
         if (pml_blk) {
             const size_t xo = (size_t)slot * p.xst, zo = (size_t)slot * p.zst;
@@ -445,7 +495,7 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
         }
         const T vxn = rvx[k] + dt * rrxp[k] * (sxx_x + szx_z);
         const T vzn = rvz[k] + dt * rrzp[k] * (szx_x + szz_z);
-        if (energy) p.We[fo + (size_t)iz * pitch + ix] = rwe[k] + (vxn * vxn + vzn * vzn);
+        if (energy) stg(WeS, boff(iz, ix), rwe[k] + (vxn * vxn + vzn * vzn));
         rvx[k] = vxn;
         rvz[k] = vzn;
     }
@@ -495,12 +545,12 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
         const int rr = ty + k * kRowsPerWave;
         const int iz = z0 + rr, ix = x0 + tx;
         if (iz >= nzt || ix >= nxt) continue;
-        const size_t g = fo + (size_t)iz * pitch + ix;
-        p.vx[g] = rvx[k];
-        p.vz[g] = rvz[k];
-        if (fsurf && iz < fpad && ix >= fpad && ix < nxt - fpad) {
-            p.szx[g] = lszx[rr + kHalo][tx + kHalo];
-            p.szz[g] = lszz[rr + kHalo][tx + kHalo];
+        const unsigned o = boff(iz, ix);
+        stg(vxS, o, rvx[k]);
+        stg(vzS, o, rvz[k]);
+        if (top && iz < fpad && ix >= fpad && ix < nxt - fpad) {
+            stg(szxS, o, lszx[rr + kHalo][tx + kHalo]);
+            stg(szzS, o, lszz[rr + kHalo][tx]);
         }
     }
 }
