@@ -99,6 +99,8 @@ __global__ __launch_bounds__(kThreads) void psv_fused_kernel(const StepParams<T>
     const bool fsurf = p.fsurf != 0;
     const bool energy = (p.flags & kEnergy) != 0;
     const bool fwi_step = (p.flags & kStore) && (it % p.fwi_dt == 0);
+    const bool unit_stride = p.fwi_dz == 1 && p.fwi_dx == 1;
+    const int tf_cur = it / p.fwi_dt - p.tf0;
     const bool pre = MODE == kAdjoint || fwi_step;
 
     // uniform bases at the start of the slot's allocation: byte offsets from
@@ -251,11 +253,13 @@ __global__ __launch_bounds__(kThreads) void psv_fused_kernel(const StepParams<T>
                 }
             }
             const int dzw = gz - p.fwi_z1, dxw = gx - p.fwi_x1;
-            if (fwi_step && dzw >= 0 && dxw >= 0 && gz < p.fwi_z2 && gx < p.fwi_x2 && dzw % p.fwi_dz == 0 &&
-                dxw % p.fwi_dx == 0) {
-                const int tf = it / p.fwi_dt - p.tf0;
+            // unit strides (the common case) need no integer division
+            const bool on = unit_stride ? true : (dzw % p.fwi_dz == 0 && dxw % p.fwi_dx == 0);
+            if (fwi_step && dzw >= 0 && dxw >= 0 && gz < p.fwi_z2 && gx < p.fwi_x2 && on) {
+                const int tf = tf_cur;
                 const size_t plane = (size_t)p.nfz * p.nfx;
-                const size_t w = (size_t)(dzw / p.fwi_dz) * p.nfx + dxw / p.fwi_dx;
+                const int zf = unit_stride ? dzw : dzw / p.fwi_dz, xf = unit_stride ? dxw : dxw / p.fwi_dx;
+                const size_t w = (size_t)zf * p.nfx + xf;
                 T* st = p.store + (size_t)slot * p.sst + (size_t)tf * 5 * plane + w;
                 if (MODE == kForward) {
                     st[0 * plane] = lvx[vr][vc] / dt;

@@ -55,7 +55,8 @@ psvfwi_status fail(psvfwi_status st, const std::string& msg)
 template <typename T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;    // elements in use
+    size_t cap = 0;  // elements allocated
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
@@ -64,17 +65,28 @@ struct DBuf {
     {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
+    // exactly count elements
     hipError_t alloc(size_t count)
     {
-        if (p && count == n) return hipSuccess;
+        if (p && count == n && count == cap) return hipSuccess;
         release();
         if (count == 0) return hipSuccess;
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-        if (e == hipSuccess) n = count;
+        if (e == hipSuccess) n = cap = count;
         else p = nullptr;
         return e;
+    }
+    // at least count elements; keeps a larger allocation (large buffers whose
+    // size changes between runs are not freed and mapped again every time)
+    hipError_t reserve(size_t count)
+    {
+        if (p && count <= cap) {
+            n = count;
+            return hipSuccess;
+        }
+        return alloc(count);
     }
 };
 
@@ -393,10 +405,10 @@ public:
         PSV_HIP(psv::launch_fill<T>(rec_.p, rec_.n, (T)0, stream_));
         const bool full = grad && store_mode_ == PSVFWI_STORE_FULL;
         const bool ckpt = grad && store_mode_ == PSVFWI_STORE_CHECKPOINT;
-        if (full) {
-            PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
-            PSV_HIP(psv::launch_fill<T>(store_.p, store_.n, (T)0, stream_));
-        }
+        // the store is not cleared: the adjoint reads exactly the entries the
+        // forward writes, and get_store zeroes the ones the window loops never
+        // write (see window_written)
+        if (full) PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
         if (ckpt) PSV_TRY(alloc_checkpoints());
         const bool snap = snap_interval_ > 0;
         if (snap) PSV_HIP(snaps_.alloc((size_t)((ntmax_ - 1) / snap_interval_ + 1) * nslots_ * fst_));
@@ -669,10 +681,20 @@ public:
         const size_t plane = (size_t)nfz_ * nfx_;
         std::vector<double> h(sst_);
         PSV_TRY(download(store_.p + (size_t)s * sst_, sst_, h.data()));
+        // the window loops run over [z1, z2) x [x1, x2): when an extent is a
+        // multiple of its stride the last row / column of the store is never
+        // written (CUDA_week2/forward_kernel_PSV_GPU.cu:803-821) and reads 0
+        const int nzw = (w_.z2 - w_.z1 + w_.dz - 1) / w_.dz, nxw = (w_.x2 - w_.x1 + w_.dx - 1) / w_.dx;
         for (int f = 0; f < 5; f++)
-            for (int t = 0; t < nft; t++)
-                std::memcpy(out + ((size_t)f * nft + t) * plane, h.data() + ((size_t)t * 5 + f) * plane,
-                            plane * sizeof(double));
+            for (int t = 0; t < nft; t++) {
+                double* o = out + ((size_t)f * nft + t) * plane;
+                const double* src = h.data() + ((size_t)t * 5 + f) * plane;
+                for (int zf = 0; zf < nfz_; zf++)
+                    for (int xf = 0; xf < nfx_; xf++) {
+                        const size_t k = (size_t)zf * nfx_ + xf;
+                        o[k] = (zf < nzw && xf < nxw) ? src[k] : 0.0;
+                    }
+            }
         return PSVFWI_OK;
     }
 
@@ -956,7 +978,7 @@ private:
     psvfwi_status alloc_store(int ntf)
     {
         sst_ = (size_t)ntf * 5 * nfz_ * nfx_;
-        PSV_HIP(store_.alloc((size_t)nslots_ * sst_));
+        PSV_HIP(store_.reserve((size_t)nslots_ * sst_));
         return PSVFWI_OK;
     }
 
