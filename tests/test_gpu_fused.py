@@ -71,14 +71,16 @@ def test_fused_checkpoint_gradient_equals_full_store():
             assert np.array_equal(a, b), k
 
 
-def test_fused_batched_uneven_shots():
-    """Shots that end before the longest one keep their own final state
-    (which of the two buffer sets holds it depends on nt)."""
-    print("fused batched uneven shots", flush=True)
+@pytest.mark.parametrize("tz", [-16, -8])
+def test_batched_uneven_shots_variants(tz):
+    """Batches with the fused kernel equal single-shot plans of the split
+    kernels: shots that end before the longest one keep their own final
+    state (which buffer set holds it depends on nt)."""
+    print(f"batched uneven shots, tile {tz}", flush=True)
     base = RC.small_problem(nx=80, nz=48, npml=8, nt=10, fdorder=4, fsurf=1, hetero=True)
     rng = np.random.default_rng(9)
     shots, obs = [], []
-    for i, nt in enumerate((150, 231, 90, 230)):
+    for i, nt in enumerate((150, 231, 90, 230, 177)):
         nrec = int(rng.integers(3, 12))
         xs = int(rng.integers(base.ppad + 5, base.nxt - base.ppad - 5))
         sig = O.create_wavelet(nt, base.dt, 1.0 + i, 25.0, 0.0)[None]
@@ -88,7 +90,7 @@ def test_fused_batched_uneven_shots():
                           rec=rec))
         obs.append(rng.standard_normal((nrec, nt)) * 1e-3)
     win = RC.full_window(base, dt=2)
-    batch = plan_of(base, "fp32", shots=shots, tz=-16)
+    batch = plan_of(base, "fp32", shots=shots, tz=tz)
     batch.set_window(win)
     batch.forward(gradient=True)
     fwd_fields = [{k: batch.field(k, i) for k in FIELDS + MEMS} for i in range(len(shots))]

@@ -178,7 +178,7 @@ def test_autotune_picks_a_tile_and_keeps_results():
     ref = O.forward(pb, prec="f32")
     plan = plan_of(pb, "fp32")
     best = plan.autotune(30)
-    assert best in (4, 8, 16, 32)
+    assert best in (4, 8, 16, 32, -8, -16)
     plan.forward()
     assert np.array_equal(plan.records(), ref["rec"].astype(np.float64))
 
