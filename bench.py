@@ -316,6 +316,7 @@ def run_c4(args):
 
     obs = {}
     plan = new_plan(groups[0], args.store)
+    plan.set_graph_steps(args.graph_steps)
     plan.set_model(lt, mt, *P.average_parameters(mt, rt))
     tile = args.tile if args.tile != 0 else plan.autotune(min(200, w.nt))
     for g in groups:
@@ -365,6 +366,7 @@ def run_c4(args):
     tp = 0.0
     sample = list(range(min(args.per_shot_sample, len(w.shots))))
     p1 = new_plan([sample[0]], args.store)
+    p1.set_graph_steps(args.graph_steps)
     p1.set_model(lam, mu, *avg0)
     p1.set_tile(tile)
     fwd_adj(p1, [sample[0]])  # warm-up
@@ -397,7 +399,7 @@ def run_c4(args):
                                       f"groups sized to {args.store_budget_gb} GB of store"
                                       if not recompute else f"checkpoint every {args.interval} steps, "
                                       "segments recomputed (recomputation not counted)"),
-                   "tile": tile},
+                   "tile": tile, "graph_steps": args.graph_steps},
         "hbm": {"alg_bytes": int(alg), "alg_GBps": round(alg / tb / 1e9, 1),
                 "frac_of_peak": round(alg / tb / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "algorithmic bytes of forward + adjoint + gradient over the wall time"},
@@ -426,6 +428,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a 1 GiB device copy (counter calibration)")
     ap.add_argument("--per-shot-sample", type=int, default=8)
+    ap.add_argument("--graph-steps", type=int, default=0,
+                    help="replay the time loop as captured HIP graphs of this many steps (0: direct launches)")
     ap.add_argument("--store-budget-gb", type=float, default=200.0,
                     help="batched workload with --store full: store memory per group of shots")
     args = ap.parse_args()

@@ -110,6 +110,7 @@ struct psvfwi_plan {
     virtual psvfwi_status set_store_mode(int, int) = 0;
     virtual psvfwi_status set_shots(int, const psvfwi_shot_desc*) = 0;
     virtual psvfwi_status set_tile(int) = 0;
+    virtual psvfwi_status set_graph_steps(int) = 0;
     virtual psvfwi_status set_snapshots(int, int) = 0;
     virtual psvfwi_status get_snapshot(int, int, double*) const = 0;
     virtual psvfwi_status snapshot_count(int, int*) const = 0;
@@ -297,6 +298,14 @@ public:
         return PSVFWI_OK;
     }
 
+    psvfwi_status set_graph_steps(int n) override
+    {
+        if (n < 0 || (n % 2) != 0) return fail(PSVFWI_E_ARG, "set_graph_steps: need 0 or an even number of steps");
+        if (n > 0) PSV_HIP(it_dev_.alloc(1));
+        graph_steps_ = n;
+        return PSVFWI_OK;
+    }
+
     // Times `steps` forward launches of the current batch for every tile
     // height and keeps the fastest.  Runs on the plan's own state buffers
     // (every forward/adjoint starts from a zeroed state, so nothing leaks).
@@ -418,13 +427,36 @@ public:
         PSV_TRY(begin_samples());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
+        const int fflags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
+        std::vector<hipGraphExec_t> graphs;
         for (int it = 0; it < ntmax_; it++) {
             const int na = active(it);
+            if (graph_steps_ > 0 && !snap && !ckpt) {
+                // a run of steps with the same active shots: replay a captured
+                // chunk of graph_steps_ steps, the step index read from it_dev_
+                int end = it;
+                while (end < ntmax_ && active(end) == na) end++;
+                if (end - it >= graph_steps_) {
+                    psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
+                    p.nslots = na;
+                    p.flags = fflags;
+                    p.tf0 = 0;
+                    hipGraphExec_t ex = nullptr;
+                    PSV_TRY(capture_steps(p, psv::kForward, +1, main_, main2_, fwd_tab_, true, &ex));
+                    graphs.push_back(ex);
+                    for (; end - it >= graph_steps_; it += graph_steps_) {
+                        PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
+                        PSV_HIP(hipGraphLaunch(ex, stream_));
+                        launches_ += graph_steps_;
+                    }
+                    if (it >= ntmax_) break;
+                }
+            }
             if (ckpt && it % ck_k_ == 0) PSV_TRY(save_checkpoint(it / ck_k_, na));
             psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
             p.it = it;
-            p.nslots = na;
-            p.flags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
+            p.nslots = active(it);
+            p.flags = fflags;
             p.tf0 = 0;
             PSV_TRY(launch_timed(p, psv::kForward, 0, main_, main2_));
             if (snap && it % snap_interval_ == 0) {
@@ -436,6 +468,7 @@ public:
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
+        for (hipGraphExec_t ex : graphs) PSV_HIP(hipGraphExecDestroy(ex));
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
@@ -573,8 +606,31 @@ public:
         PSV_TRY(begin_samples());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
+        std::vector<hipGraphExec_t> graphs;
         if (!ckpt) {
-            for (int it = ntmax_ - 1; it >= 0; it--) PSV_TRY(adjoint_step(it, aflags, 0));
+            for (int it = ntmax_ - 1; it >= 0; it--) {
+                if (graph_steps_ > 0) {
+                    const int na = active(it);
+                    int end = it;  // steps it, it-1, ..., end+1 have the same active shots
+                    while (end >= 0 && active(end) == na) end--;
+                    if (it - end >= graph_steps_) {
+                        psv::StepParams<T> p = base_params(main_, adj_tab_, false);
+                        p.nslots = na;
+                        p.flags = aflags;
+                        p.tf0 = 0;
+                        hipGraphExec_t ex = nullptr;
+                        PSV_TRY(capture_steps(p, psv::kAdjoint, -1, main_, main2_, adj_tab_, false, &ex));
+                        graphs.push_back(ex);
+                        for (; it - end >= graph_steps_; it -= graph_steps_) {
+                            PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
+                            PSV_HIP(hipGraphLaunch(ex, stream_));
+                            launches_ += graph_steps_;
+                        }
+                        if (it < 0) break;
+                    }
+                }
+                PSV_TRY(adjoint_step(it, aflags, 0));
+            }
         } else {
             PSV_TRY(alloc_state(replay_));
             if (fused()) {
@@ -604,6 +660,7 @@ public:
         }
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
+        for (hipGraphExec_t ex : graphs) PSV_HIP(hipGraphExecDestroy(ex));
         float ms = 0;
         PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
         last_ms_ = ms;
@@ -741,6 +798,38 @@ private:
         DBuf<int> tinj_off, tinj_idx, trec_off, trec_idx;
         int built_tz = -1;
     };
+
+    // Captures graph_steps_ consecutive steps starting at the step held in
+    // it_dev_ (step k of the chunk is *it_dev_ + dir * k) into an executable
+    // graph.  An even count leaves the ping-pong buffers of the fused step in
+    // their original roles.  The chunk is captured on the plan's stream and
+    // holds only kernel launches.
+    psvfwi_status capture_steps(psv::StepParams<T> p0, int mode, int dir, State<T>& st, State<T>& alt,
+                                const Tables& tab, bool fwd_sig, hipGraphExec_t* out)
+    {
+        hipGraph_t g = nullptr;
+        PSV_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        psvfwi_status rc = PSVFWI_OK;
+        for (int k = 0; k < graph_steps_ && rc == PSVFWI_OK; k++) {
+            psv::StepParams<T> p = base_params(st, tab, fwd_sig);
+            p.nslots = p0.nslots;
+            p.flags = p0.flags;
+            p.tf0 = p0.tf0;
+            p.it = dir * k;
+            p.it_dev = it_dev_.p;
+            rc = launch_plain(p, mode, st, alt);
+        }
+        const hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (rc != PSVFWI_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PSV_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PSV_HIP(ei);
+        return PSVFWI_OK;
+    }
 
     bool in_grid(int z, int x) const { return z >= 0 && z < g_.nzt && x >= 0 && x < g_.nxt; }
 
@@ -1225,6 +1314,8 @@ private:
     psvfwi_grid_desc g_;
     int fpad_ = 1, ppad_ = 0, tp_ = 2, pitch_ = 0, rows_ = 0, org_ = 0, pnx_ = 0, pnz_ = 0, tz_ = 16;
     bool parity_ = false;
+    int graph_steps_ = 0;  // > 0: time loops replay captured chunks of this many steps
+    DBuf<int> it_dev_;
     size_t fst_ = 0, xst_ = 0, zst_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -1367,6 +1458,12 @@ psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile)
 {
     PSV_PLAN(plan);
     return plan->autotune(steps, best_tile);
+}
+
+psvfwi_status psvfwi_set_graph_steps(psvfwi_plan* plan, int steps)
+{
+    if (!plan) return fail(PSVFWI_E_ARG, "null plan");
+    return plan->set_graph_steps(steps);
 }
 
 psvfwi_status psvfwi_set_snapshots(psvfwi_plan* plan, int interval, int field)

@@ -34,7 +34,7 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing", "psvfwi_kernel_timing",
            "psvfwi_parabola_step", "psvfwi_holberg", "psvfwi_cpml_profiles",
            "psvfwi_average_parameters", "psvfwi_ricker", "psvfwi_set_snapshots",
-           "psvfwi_snapshot_count", "psvfwi_get_snapshot")
+           "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps")
 
 
 class PsvfwiError(RuntimeError):
@@ -168,6 +168,10 @@ class Plan:
         _check(lib().psvfwi_set_shots(self._h, len(descs), arr))
         self._shots = [dict(nt=int(s["nt"]), nrec=len((s.get("rec") or {"x": []})["x"]))
                        for s in shots]
+
+    def set_graph_steps(self, steps):
+        """Replay time steps as captured HIP graphs of `steps` launches (0: off)."""
+        _check(lib().psvfwi_set_graph_steps(self._h, int(steps)))
 
     def set_snapshots(self, interval, field="szz"):
         _check(lib().psvfwi_set_snapshots(self._h, int(interval), FIELDS[field] if isinstance(field, str) else int(field)))

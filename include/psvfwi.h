@@ -123,6 +123,16 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
  * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */
 psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile);
 
+/* Graph-captured time loop: with steps > 0 (even), forward and adjoint
+ * runs replay HIP graphs of `steps` consecutive step launches (the step index
+ * is read from device memory) over every run of steps in which the set of
+ * active shots does not change; checkpointed stores and snapshots keep the
+ * direct launches.  0 (default) launches every step directly.  Results are
+ * identical either way; per-kernel timing samples are taken only from direct
+ * launches.  (No counterpart in the reference, which launches two kernels per
+ * step and synchronizes after each, CUDA_week2/forward_kernel_PSV_GPU.cu:585-600.) */
+psvfwi_status psvfwi_set_graph_steps(psvfwi_plan* plan, int steps);
+
 /* Snapshots of one wavefield (field 0 vx, 1 vz, 2 sxx, 3 szx, 4 szz) kept on
  * the device after every step it with it % interval == 0 of the next forward
  * runs (interval 0 switches them off).  The reference writes szz there
