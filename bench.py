@@ -410,12 +410,57 @@ def run_c4(args):
     }
 
 
+def run_c3mem(args):
+    """Memory test: 2000 x 1000 cells, nt 8000, FP32, one shot, gradient with
+    the source wavefield rebuilt from checkpoints every --interval steps.
+    Reports the device memory in use (hipMemGetInfo through torch) and what
+    the full decimated store would need."""
+    import torch
+    from cuda_fwi_amd import workloads as WL
+    w = WL.c3mem()
+    if args.nt:
+        w.nt = args.nt
+        w.shots[0]["nt"] = args.nt
+    fwi_dt = args.fwi_dt
+    window = w.model_window(dt=fwi_dt)
+    torch.cuda.init()
+    free0, total = torch.cuda.mem_get_info()
+    plan = make_plan(w, args.precision, window, "checkpoint", args.interval, args.tile)
+    esize = 4 if args.precision == "fp32" else 8
+    used = []
+    _sync()
+    t0 = time.perf_counter()
+    plan.forward(energy=False, gradient=True)
+    used.append(total - torch.cuda.mem_get_info()[0])
+    plan.misfit()
+    plan.adjoint(energy=False, gradient=True)
+    _sync()
+    T = time.perf_counter() - t0
+    used.append(total - torch.cuda.mem_get_info()[0])
+    cells = w.interior_cells()
+    full_store = 5 * w.nx * w.nz * (1 + (w.nt - 1) // 1) * esize
+    return {
+        "metric": METRIC, "value": round(2.0 * cells * w.nt / T / 1e9, 3), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(T * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if esize == 4 else "f64",
+        "data": "synthetic: 8 flat layers", "config": {
+            "workload": f"2000x1000 dx5 order-4 npml40, nt {w.nt}, 1 shot, fwd+adj+grad, checkpoint every "
+                        f"{args.interval} steps, fwi_dt {fwi_dt}", "tile": plan.tile},
+        "memory": {"device_used_before_GB": round((total - free0) / 1e9, 2),
+                   "device_used_peak_GB": round(max(used) / 1e9, 2),
+                   "device_total_GB": round(total / 1e9, 1),
+                   "full_store_every_step_GB": round(full_store / 1e9, 1),
+                   "note": "device memory in use after the forward and after the adjoint (hipMemGetInfo); "
+                           "the store of every step (fwi_dt 1) would not fit"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c3mem"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--store", default="full", choices=["full", "checkpoint"])
     ap.add_argument("--interval", type=int, default=200)
@@ -443,7 +488,8 @@ def main():
     if args.calibrate_copy:
         print(json.dumps(calibrate_copy()), flush=True)
         return
-    line = run_c2(args) if args.workload == "c2" else run_c4(args)
+    run = {"c2": run_c2, "c4": run_c4, "c3mem": run_c3mem}[args.workload]
+    line = run(args)
     print(json.dumps(line), flush=True)
 
 

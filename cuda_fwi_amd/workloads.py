@@ -14,7 +14,8 @@ lam = vp^2 rho - 2 mu, as FDMedium::velocity_to_lame).
       vs = vp/1.73, Gardner density, 3 % Gaussian perturbation (50 m),
       12 Hz Ricker at (500, 2), 400 receivers, dt 0.4 ms, nt 5000
   c3  64 x 64, nt 600, 1 source, 32 receivers, random smooth model
-      (differentiation check); c3mem 2000 x 1000, nt 8000 (memory test)
+      (differentiation check); c3mem 2000 x 1000, nt 8000, 8 flat layers,
+      start = layers smoothed over 50 m (memory test)
   c4  1500 x 600, 64 shots: nt U{3000..8000}, receivers U{50..400},
       source x U[100, 1400], layered with a 60 degree dipping fault
   c5  2400 x 800, 128 shots x 600 receivers, nt 6000, 8 Hz, layers + two
@@ -157,7 +158,10 @@ def c3mem():
     vp = c2_layers(nx, nz)
     vs = vp / 1.73
     shots = [dict(nt=8000, src=dict(x=[1000], z=[2], comp=[2]), rec=_line(nx, 400), amp=1.0)]
-    return Workload("c3mem", nx, nz, dx, 4e-4, 8000, 40, 4, 0, 12.0, vp, vs, _gardner(vp), shots)
+    w = Workload("c3mem", nx, nz, dx, 4e-4, 8000, 40, 4, 0, 12.0, vp, vs, _gardner(vp), shots)
+    sm = gaussian_filter(vp, 10.0, mode="nearest")
+    w.background = (sm, sm / 1.73, _gardner(sm))
+    return w
 
 
 def _layered_with_fault(nx, nz, dx, seed, nlayer=8, vmin=1800.0, vmax=4200.0, offset_m=50.0,
