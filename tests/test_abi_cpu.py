@@ -64,3 +64,34 @@ def test_library_setup_functions_equal_the_oracle():
     for a, b in zip(P.average_parameters(mu, rho), O.average_parameters(mu, rho)):
         assert np.array_equal(a, b)
     assert np.array_equal(P.ricker(3000, 4e-4, 2.0, 12.0, 0.01), O.create_wavelet(3000, 4e-4, 2.0, 12.0, 0.01))
+
+
+def test_plan_create_rejects_bad_grids_before_touching_a_device():
+    """Argument errors come back as status codes with a message (the
+    reference exits instead); these checks run before any device call."""
+    from cuda_fwi_amd import psvfwi as P
+    lib = ctypes.CDLL(P.LIB_PATH)
+    lib.psvfwi_last_error.restype = ctypes.c_char_p
+
+    def create(**kw):
+        g = P.GridDesc()
+        base = dict(nzt=60, nxt=80, fdorder=4, npml=8, fsurf=1, dt=5e-4, dx=5.0, dz=5.0)
+        base.update(kw)
+        for k, v in base.items():
+            setattr(g, k, v)
+        out = ctypes.c_void_p()
+        st = lib.psvfwi_plan_create(ctypes.byref(g), kw.get("prec", 0), ctypes.byref(out))
+        return st, lib.psvfwi_last_error().decode()
+
+    assert create(fdorder=3)[0] == -3                       # PSVFWI_E_ORDER
+    st, msg = create(nzt=4)
+    assert st == -1 and "smaller than the stencil" in msg
+    st, msg = create(npml=40)
+    assert st == -1 and "CPML wider" in msg
+    st, msg = create(dt=0.0)
+    assert st == -1 and "dt, dx, dz" in msg
+    st, msg = create(prec=7)
+    assert st == -1 and "precision" in msg
+    st, msg = create(nzt=30000, nxt=30000, npml=8)
+    assert st == -1 and "too large" in msg
+    assert lib.psvfwi_plan_create(None, 0, None) == -1
