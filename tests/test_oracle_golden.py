@@ -314,3 +314,44 @@ def test_gradient_brute_force_tiny():
     # the last decimated row/column is never written (strict < in the window loops)
     assert nfz > len(zs) or nfx > len(xs)
     assert np.all(adj["grad_lam"][len(zs):, :] == 0) and np.all(adj["grad_lam"][:, len(xs):] == 0)
+
+
+def _lag(a, b):
+    """Delay of trace b behind trace a in samples (cross-correlation peak,
+    refined by a parabola through its three top samples)."""
+    c = np.correlate(b, a, "full")
+    k = int(np.argmax(c))
+    y0, y1, y2 = c[k - 1], c[k], c[k + 1]
+    return k - (len(a) - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+
+
+@pytest.mark.parametrize("fdorder,tol", [(2, 0.03), (4, 0.01)])
+def test_homogeneous_p_and_s_speeds(fdorder, tol):
+    """A vertical point force in a homogeneous medium radiates P waves along
+    the vertical and S waves along the horizontal.  The move-out between two
+    receivers on each axis gives vp = sqrt((lam + 2 mu) / rho) and
+    vs = sqrt(mu / rho).  dx != dz, so a swapped grid step fails; a dropped
+    2 mu term (vp -> 1444 m/s) or a dropped or sign-flipped second Holberg
+    coefficient (speeds x 9/8 or x 5/4) fail too.  The tolerance covers the
+    grid dispersion of each order at 4.6 points per shortest S wavelength."""
+    npml, n = 20, 200
+    fpad = fdorder // 2
+    ppad = npml + fpad
+    nxt = nzt = n + 2 * ppad
+    dx, dz, dt, fc, nt = 5.0, 4.0, 4e-4, 25.0, 1400
+    vp, vs, rho = 2500.0, 1443.0, 2000.0
+    lam0, mu0 = O.velocity_to_lame(vp, vs, rho)
+    lam, mu, rh = (np.full((nzt, nxt), v) for v in (lam0, mu0, rho))
+    z0 = x0 = ppad + 60
+    r1, r2 = 40, 100
+    rec = dict(x=[x0, x0, x0 + r1, x0 + r2], z=[z0 + r1, z0 + r2, z0, z0], comp=[2] * 4, shot=[0] * 4)
+    pb = O.Problem(nt=nt, nzt=nzt, nxt=nxt, fdorder=fdorder, npml=npml, dt=dt, dx=dx, dz=dz, fsurf=0,
+                   hc=O.holbergcoeff(fdorder, 1), lam=lam, mu=mu, rho=rh,
+                   pml=O.cpml_pro(npml, dx, fc, 2.0, 1.0, 1e-3, 2750.0, dt),
+                   src=dict(x=[x0], z=[z0], comp=[2], shot=[0]),
+                   src_signal=O.create_wavelet(nt, dt, 1.0, fc, 0.0, 1)[None], rec=rec)
+    d = O.forward(pb)["rec"]
+    cp = (r2 - r1) * dz / (_lag(d[0], d[1]) * dt)
+    cs = (r2 - r1) * dx / (_lag(d[2], d[3]) * dt)
+    assert abs(cp / vp - 1) < tol, cp
+    assert abs(cs / vs - 1) < tol, cs
