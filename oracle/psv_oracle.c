@@ -351,15 +351,52 @@ double psvo_calc_opt_step(const double L2[3], const double sl[3])
     return -x1 / (2.0 * x2);
 }
 
+/* Chain rule from the coefficient arrays to lam, mu, rho (not in the
+ * reference): mu_zx and rho_zp / rho_xp are the averages of
+ * CUDA_week2/fd_medium.cu:128-159 (psvo_average_parameters), so
+ *   d mu_zx[o] / d mu[q]  = mu_zx[o]^2 / (4 mu[q]^2) for its 4 corners q,
+ *   d rho_zp[o] / d rho[q] = -rho_zp[o]^2 / 2 for (iz, ix) and (iz + 1, ix),
+ *   d rho_xp[o] / d rho[q] = -rho_xp[o]^2 / 2 for (iz, ix) and (iz, ix + 1);
+ * an average the guards set to zero has zero derivative.  Contributions to
+ * a cell are added in the order o = (iz, ix), (iz, ix - 1), (iz - 1, ix),
+ * (iz - 1, ix - 1). */
+void psvo_chain_gradient(int nzt, int nxt, const double* mu, const double* mu_zx,
+                         const double* rho_zp, const double* rho_xp,
+                         const double* g_lam, const double* g_mu, const double* g_mu_zx,
+                         const double* g_rho_zp, const double* g_rho_xp,
+                         double* out_lam, double* out_mu, double* out_rho)
+{
+    const int ax = nxt - 1;
+    for (int iz = 0; iz < nzt; iz++) {
+        for (int ix = 0; ix < nxt; ix++) {
+            const size_t q = (size_t)iz * nxt + ix;
+            double gm = g_mu[q], gr = 0.0;
+            for (int k = 0; k < 4; k++) {
+                const int oz = iz - (k >> 1), ox = ix - (k & 1);
+                if (oz < 0 || ox < 0 || oz >= nzt - 1 || ox >= nxt - 1) continue;
+                const size_t o = (size_t)oz * ax + ox;
+                if (mu_zx[o] != 0.0) gm += g_mu_zx[o] * (mu_zx[o] * mu_zx[o] / (4.0 * mu[q] * mu[q]));
+                if (ox == ix) gr += g_rho_zp[o] * (-0.5 * rho_zp[o] * rho_zp[o]);   /* rows iz, iz + 1 */
+                if (oz == iz) gr += g_rho_xp[o] * (-0.5 * rho_xp[o] * rho_xp[o]);   /* columns ix, ix + 1 */
+            }
+            out_lam[q] = g_lam[q];
+            out_mu[q] = gm;
+            out_rho[q] = gr;
+        }
+    }
+}
+
 /* ------------------------------------------------------------------ */
 #define REAL double
 #define SUF _d
 #include "psv_oracle_prop.inc"
+#include "psv_oracle_discrete.inc"
 #undef REAL
 #undef SUF
 
 #define REAL float
 #define SUF _f
 #include "psv_oracle_prop.inc"
+#include "psv_oracle_discrete.inc"
 #undef REAL
 #undef SUF

@@ -145,6 +145,28 @@ int psvo_adjoint_f(const psvo_cfg* c, int ishot,
                    const float* store,
                    float* grad_lam, float* grad_mu, float* grad_rho);
 
+/* ---- discrete adjoint (an extension: the exact gradient of the misfit of
+ * this scheme, psv_oracle_discrete.inc) ---- */
+int psvo_discrete_gradient_d(const psvo_cfg* c, int ishot,
+                             const double* lam, const double* mu, const double* mu_zx,
+                             const double* rho_zp, const double* rho_xp, const double* pml,
+                             const psvo_seis* src, const double* src_signal,
+                             const psvo_seis* rec, const double* res,
+                             double* g_lam, double* g_mu, double* g_mu_zx,
+                             double* g_rho_zp, double* g_rho_xp, double* adj);
+int psvo_discrete_gradient_f(const psvo_cfg* c, int ishot,
+                             const float* lam, const float* mu, const float* mu_zx,
+                             const float* rho_zp, const float* rho_xp, const float* pml,
+                             const psvo_seis* src, const float* src_signal,
+                             const psvo_seis* rec, const float* res,
+                             float* g_lam, float* g_mu, float* g_mu_zx,
+                             float* g_rho_zp, float* g_rho_xp, float* adj);
+void psvo_chain_gradient(int nzt, int nxt, const double* mu, const double* mu_zx,
+                         const double* rho_zp, const double* rho_xp,
+                         const double* g_lam, const double* g_mu, const double* g_mu_zx,
+                         const double* g_rho_zp, const double* g_rho_xp,
+                         double* out_lam, double* out_mu, double* out_rho);
+
 #ifdef __cplusplus
 }
 #endif

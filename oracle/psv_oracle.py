@@ -284,3 +284,46 @@ def adjoint(pb: Problem, adj_src: dict, adj_signal: np.ndarray, store: np.ndarra
     f["grad_lam"], f["grad_mu"], f["grad_rho"] = g
     f["mem"] = mem
     return f
+
+
+def discrete_gradient(pb: Problem, res: np.ndarray, ishot=0, prec="f64"):
+    """Exact gradient of the discrete misfit sum 0.5 dt (d - d_obs)^2 of shot
+    `ishot` with respect to the coefficient arrays (discrete adjoint, an
+    extension; psv_oracle_discrete.inc).  res [nrec, nt] = d - d_obs for the
+    receivers of pb.rec.  Returns dict g_lam, g_mu [nzt, nxt], g_mu_zx,
+    g_rho_zp, g_rho_xp [nzt-1, nxt-1] and adj (5, nzt, nxt): the adjoint of
+    the initial vx, vz, sxx, szx, szz."""
+    dt_ = _dtype(prec)
+    fn = lib().psvo_discrete_gradient_d if prec == "f64" else lib().psvo_discrete_gradient_f
+    n, na = (pb.nzt, pb.nxt), (pb.nzt - 1, pb.nxt - 1)
+    g = {k: np.zeros(n, dt_) for k in ("g_lam", "g_mu")}
+    g.update({k: np.zeros(na, dt_) for k in ("g_mu_zx", "g_rho_zp", "g_rho_xp")})
+    adj = np.zeros((5,) + n, dt_)
+    keep = []
+    src = Problem._seis(pb.src, keep)
+    rec = Problem._seis(pb.rec, keep)
+    cast = lambda a: np.ascontiguousarray(a, dt_)
+    mats = [cast(pb.lam), cast(pb.mu), cast(pb.mu_zx), cast(pb.rho_zp), cast(pb.rho_xp)]
+    ssig = cast(pb.src_signal) if pb.src_signal is not None else np.zeros(1, dt_)
+    r = cast(res) if np.size(res) else np.zeros(1, dt_)
+    cfg = pb.cfg()
+    st = fn(C.byref(cfg), C.c_int(ishot), *[_p(m) for m in mats], _p(cast(pb.pml)),
+            C.byref(src) if src else None, _p(ssig), C.byref(rec) if rec else None, _p(r),
+            *[_p(g[k]) for k in ("g_lam", "g_mu", "g_mu_zx", "g_rho_zp", "g_rho_xp")], _p(adj))
+    if st != 0:
+        raise ValueError(f"oracle discrete gradient rejected its arguments (status {st})")
+    g["adj"] = adj
+    return g
+
+
+def chain_gradient(mu, mu_zx, rho_zp, rho_xp, g):
+    """lam / mu / rho gradients from the coefficient gradients g (as returned
+    by discrete_gradient), for averages made by average_parameters."""
+    nzt, nxt = mu.shape
+    out = [np.zeros((nzt, nxt)) for _ in range(3)]
+    a = lambda x: np.ascontiguousarray(x, np.float64)
+    lib().psvo_chain_gradient(C.c_int(nzt), C.c_int(nxt), _p(a(mu)), _p(a(mu_zx)), _p(a(rho_zp)),
+                              _p(a(rho_xp)), *[_p(a(g[k])) for k in ("g_lam", "g_mu", "g_mu_zx",
+                                                                      "g_rho_zp", "g_rho_xp")],
+                              *[_p(o) for o in out])
+    return out
