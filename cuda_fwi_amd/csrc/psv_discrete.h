@@ -1,0 +1,56 @@
+// psv_discrete.h -- launch parameters of the discrete-adjoint step kernels.
+//
+// The discrete adjoint is the exact gradient of the misfit
+// sum 0.5 dt (d - d_obs)^2 of this scheme with respect to the coefficient
+// arrays lam, mu, mu_zx, rho_zp, rho_xp (an extension: the reference's
+// gradient is the correlation of adjoint_kernel_PSV_GPU.cu:830-863).  One
+// reverse time step is: receiver adjoint sources, the transposed velocity
+// half-step (weights, then gather into the stress adjoints), the transposed
+// stress mirror, the transposed stress half-step (weights, then gather into
+// the velocity adjoints).  The forward quantities each step needs are
+// recomputed from the forward states before and after the step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace psv {
+
+template <typename T>
+struct DiscreteParams {
+    int nzt, nxt, pitch;
+    int ppad, npml, tp, fsurf, pnx, pnz, nslots, it;
+    T dt, dxi, dzi, hc1, hc2;
+    // model and CPML profile (full-grid arrays guard-padded as in StepParams)
+    const T *lam, *mu, *mu_zx, *rho_zp, *rho_xp;
+    const T* pml;
+    size_t fst, xst, zst;  // per-slot strides: full-grid arrays, x-type and z-type CPML arrays
+    // forward state before the step: velocities and both CPML sets
+    const T *pre_vx, *pre_vz;
+    const T *pre_mvxx, *pre_mvzx, *pre_mvxz, *pre_mvzz;
+    const T *pre_msxxx, *pre_mszxx, *pre_mszzz, *pre_mszxz;
+    // forward state after the step: its stresses
+    const T *post_sxx, *post_szx, *post_szz;
+    // adjoint state: fields and the adjoints of the CPML memories
+    T *vbx, *vbz, *sbxx, *sbzx, *sbzz;
+    T *bvxx, *bvzx, *bvxz, *bvzz;
+    T *bsxxx, *bszxx, *bszzz, *bszxz;
+    // weights of the derivative terms (full-grid layout, zero outside the interior)
+    T *w0, *w1, *w2, *w3;
+    // coefficient gradients (full-grid layout)
+    T *g_lam, *g_mu, *g_mu_zx, *g_rho_zp, *g_rho_xp;
+    // receiver adjoint sources: groups of co-located receivers per slot
+    const int* grp;      // 5 ints per group: z, x, comp, first entry, count
+    const int* grp_off;  // [nslots + 1]
+    const int* grp_row;  // receiver row (within the shot) of each entry
+    const T* sig;        // residuals, slot s at sig_off[s], [nrec_s][nt_s]
+    const size_t* sig_off;
+    const int* shot_nt;
+    int max_groups;
+};
+
+// One reverse time step of the active slots (p.nslots) at time index p.it.
+template <typename T>
+hipError_t launch_discrete_step(const DiscreteParams<T>& p, int order, hipStream_t s);
+
+}  // namespace psv

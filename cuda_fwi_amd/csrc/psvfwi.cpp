@@ -14,6 +14,11 @@
 // (wavefields + CPML memory) every k steps; the adjoint then replays each
 // segment forward into a small segment store just before it needs it
 // (PSVFWI_STORE_CHECKPOINT).  Both give the same gradient bit for bit.
+// Discrete gradient (PSVFWI_GRADIENT_DISCRETE, an extension): checkpoints of
+// the full state every k steps; the adjoint replays each segment keeping
+// every state of it and runs the transposed step (psv_discrete.hip) back
+// through the segment, accumulating the exact misfit gradient with respect to
+// the five coefficient arrays.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +30,7 @@
 #include <vector>
 
 #include "psv_aux.h"
+#include "psv_discrete.h"
 #include "psv_step.h"
 #include "psvfwi.h"
 
@@ -125,6 +131,9 @@ struct psvfwi_plan {
     virtual psvfwi_status get_field(int, int, double*) const = 0;
     virtual psvfwi_status get_store(int, double*) const = 0;
     virtual psvfwi_status get_gradient(int, double*, double*, double*) const = 0;
+    virtual psvfwi_status set_gradient_mode(int, int) = 0;
+    virtual psvfwi_status get_coefficient_gradient(int, double*, double*, double*, double*, double*) const = 0;
+    virtual psvfwi_status get_model_gradient(int, double*, double*, double*) const = 0;
     virtual psvfwi_status timing(double*, long*) const = 0;
     virtual psvfwi_status kernel_timing(int, double*, double*, int*) const = 0;
 };
@@ -193,6 +202,7 @@ public:
         if (!prof) return fail(PSVFWI_E_ARG, "set_pml: null profiles");
         std::vector<T> h((size_t)6 * tp_);
         for (size_t i = 0; i < h.size(); i++) h[i] = (T)prof[i];
+        h_pml_.assign(prof, prof + h.size());
         PSV_HIP(hipMemcpyAsync(pml_.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         have_pml_ = true;
@@ -218,6 +228,12 @@ public:
             PSV_HIP(hipMemcpyAsync(dst[a]->p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
             PSV_HIP(hipStreamSynchronize(stream_));
         }
+        // host copies for the chain rule of the discrete gradient
+        h_mu_.assign(mu, mu + (size_t)nzt * nxt);
+        const size_t na = (size_t)(nzt - 1) * (nxt - 1);
+        h_mu_zx_.assign(mu_zx, mu_zx + na);
+        h_rho_zp_.assign(rho_zp, rho_zp + na);
+        h_rho_xp_.assign(rho_xp, rho_xp + na);
         have_model_ = true;
         return PSVFWI_OK;
     }
@@ -247,6 +263,18 @@ public:
             return fail(PSVFWI_E_ARG, "set_store_mode: checkpoint interval must be >= 1");
         store_mode_ = mode;
         ck_k_ = interval;
+        fwd_grad_done_ = false;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_gradient_mode(int mode, int interval) override
+    {
+        if (mode != PSVFWI_GRADIENT_REFERENCE && mode != PSVFWI_GRADIENT_DISCRETE)
+            return fail(PSVFWI_E_ARG, "set_gradient_mode: unknown mode");
+        if (mode == PSVFWI_GRADIENT_DISCRETE && interval < 1)
+            return fail(PSVFWI_E_ARG, "set_gradient_mode: segment length must be >= 1");
+        grad_mode_ = mode;
+        if (mode == PSVFWI_GRADIENT_DISCRETE) disc_k_ = interval;
         fwd_grad_done_ = false;
         return PSVFWI_OK;
     }
@@ -402,7 +430,8 @@ public:
         PSV_TRY(ready());
         const bool energy = flags & PSVFWI_ENERGY;
         const bool grad = (flags & PSVFWI_GRADIENT) != 0;
-        if (grad && !win_on_) return fail(PSVFWI_E_STATE, "forward: PSVFWI_GRADIENT needs a window");
+        const bool discrete = grad && grad_mode_ == PSVFWI_GRADIENT_DISCRETE;
+        if (grad && !discrete && !win_on_) return fail(PSVFWI_E_STATE, "forward: PSVFWI_GRADIENT needs a window");
         PSV_TRY(alloc_state(main_));
         PSV_TRY(zero_state(main_));
         if (fused()) {
@@ -412,13 +441,14 @@ public:
         PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
         PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
         PSV_HIP(psv::launch_fill<T>(rec_.p, rec_.n, (T)0, stream_));
-        const bool full = grad && store_mode_ == PSVFWI_STORE_FULL;
-        const bool ckpt = grad && store_mode_ == PSVFWI_STORE_CHECKPOINT;
+        const bool full = grad && !discrete && store_mode_ == PSVFWI_STORE_FULL;
+        const bool ckpt = grad && (discrete || store_mode_ == PSVFWI_STORE_CHECKPOINT);
+        const int kint = discrete ? disc_k_ : ck_k_;
         // the store is not cleared: the adjoint reads exactly the entries the
         // forward writes, and get_store zeroes the ones the window loops never
         // write (see window_written)
         if (full) PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
-        if (ckpt) PSV_TRY(alloc_checkpoints());
+        if (ckpt) PSV_TRY(alloc_checkpoints(kint));
         const bool snap = snap_interval_ > 0;
         if (snap) PSV_HIP(snaps_.alloc((size_t)((ntmax_ - 1) / snap_interval_ + 1) * nslots_ * fst_));
         snaps_taken_ = snap;
@@ -452,7 +482,7 @@ public:
                     if (it >= ntmax_) break;
                 }
             }
-            if (ckpt && it % ck_k_ == 0) PSV_TRY(save_checkpoint(it / ck_k_, na));
+            if (ckpt && it % kint == 0) PSV_TRY(save_checkpoint(it / kint, na));
             psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
             p.it = it;
             p.nslots = active(it);
@@ -476,6 +506,8 @@ public:
         ran_ = true;
         fwd_done_ = true;
         fwd_grad_done_ = grad;
+        fwd_disc_ = discrete;
+        ck_used_k_ = kint;
         store_loaded_ = full;
         // with the fused kernel a shot that ended early holds its final state in
         // the buffer set it last wrote: main_ when ntmax - nt_s is even
@@ -574,7 +606,10 @@ public:
         PSV_TRY(ready());
         const bool energy = flags & PSVFWI_ENERGY;
         const bool grad = (flags & PSVFWI_GRADIENT) != 0;
+        if (grad && grad_mode_ == PSVFWI_GRADIENT_DISCRETE) return adjoint_discrete();
         if (grad && !win_on_) return fail(PSVFWI_E_STATE, "adjoint: PSVFWI_GRADIENT needs a window");
+        if (grad && fwd_disc_)
+            return fail(PSVFWI_E_STATE, "adjoint: the forward ran for the discrete gradient; run it again");
         if (grad && !fwd_grad_done_)
             return fail(PSVFWI_E_STATE, "adjoint: run forward with PSVFWI_GRADIENT (or set_store) first");
         for (int i = 0; i < nslots_; i++)
@@ -637,7 +672,7 @@ public:
                 PSV_TRY(alloc_state(replay2_));
                 PSV_TRY(zero_state(replay2_));
             }
-            const int kk = ck_k_;
+            const int kk = ck_used_k_;
             const int nseg = (ntmax_ + kk - 1) / kk;
             const int seg_cap = (kk + w_.dt - 1) / w_.dt + 1;
             PSV_TRY(alloc_store(seg_cap));
@@ -671,7 +706,221 @@ public:
         return PSVFWI_OK;
     }
 
+    // Discrete gradient: per segment (last first), restore its checkpoint,
+    // replay it keeping the state before every step and after the last, then
+    // run the transposed steps back through it.
+    psvfwi_status adjoint_discrete()
+    {
+        if (!fwd_grad_done_ || !fwd_disc_)
+            return fail(PSVFWI_E_STATE, "adjoint: run forward with PSVFWI_GRADIENT in the discrete gradient mode first");
+        for (int i = 0; i < nslots_; i++)
+            if (!adj_ready_[(size_t)i] && !shots_[(size_t)i].rx.empty())
+                return fail(PSVFWI_E_STATE, "adjoint: no adjoint sources (call misfit or set_adjoint_sources)");
+        if (nexpl_ > 0) return fail(PSVFWI_E_ARG, "adjoint: the discrete gradient takes vx / vz point sources only");
+        if (g_.npml > 0 && (h_pml_[(size_t)g_.npml] != 0.0 || h_pml_[(size_t)3 * tp_ + g_.npml] != 0.0))
+            return fail(PSVFWI_E_ARG, "adjoint: the discrete gradient needs a CPML profile with a = a_half = 0 at "
+                                      "index npml, as psvfwi_cpml_profiles makes");
+        const int kk = ck_used_k_;
+        PSV_TRY(alloc_state(adj_));
+        PSV_TRY(zero_state(adj_));
+        const size_t nf = (size_t)nslots_ * fst_;
+        for (int i = 0; i < 4; i++) {
+            PSV_HIP(dw_[i].alloc(nf));
+            PSV_HIP(psv::launch_fill<T>(dw_[i].p, nf, (T)0, stream_));
+        }
+        for (int i = 0; i < 5; i++) {
+            PSV_HIP(dg_[i].alloc(nf));
+            PSV_HIP(psv::launch_fill<T>(dg_[i].p, nf, (T)0, stream_));
+        }
+        PSV_TRY(alloc_state(replay_));
+        if (fused()) {
+            PSV_TRY(alloc_state(replay2_));
+            PSV_TRY(zero_state(replay2_));
+        }
+        PSV_HIP(seg_.reserve((size_t)(std::min(kk, ntmax_) + 1) * state_elems()));
+        // receiver groups of each slot, for the adjoint sources
+        const std::vector<int>& go = adj_tab_.grp_off;
+        int maxg = 0;
+        for (int sl = 0; sl < nslots_; sl++) maxg = std::max(maxg, go[(size_t)sl + 1] - go[(size_t)sl]);
+        PSV_HIP(dgrp_off_.alloc(go.size()));
+        PSV_HIP(hipMemcpy(dgrp_off_.p, go.data(), go.size() * sizeof(int), hipMemcpyHostToDevice));
+
+        PSV_HIP(hipEventRecord(ev0_, stream_));
+        launches_ = 0;
+        const int nseg = (ntmax_ + kk - 1) / kk;
+        for (int c = nseg - 1; c >= 0; c--) {
+            const int t0 = c * kk, t1 = std::min(t0 + kk, ntmax_);
+            PSV_TRY(restore_checkpoint(c, active(t0)));
+            PSV_TRY(state_copy<true>(seg_.p, active(t0), replay_));
+            for (int it = t0; it < t1; it++) {
+                psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
+                p.it = it;
+                p.nslots = active(it);
+                p.flags = 0;
+                p.We = nullptr;
+                PSV_TRY(launch_plain(p, psv::kForward, replay_, replay2_));
+                PSV_TRY(state_copy<true>(seg_.p + (size_t)(it - t0 + 1) * state_elems(), active(it), replay_));
+                launches_++;
+            }
+            for (int it = t1 - 1; it >= t0; it--) {
+                psv::DiscreteParams<T> d = dparams(it, entry(it - t0), entry(it - t0 + 1));
+                d.max_groups = maxg;
+                PSV_HIP(psv::launch_discrete_step<T>(d, g_.fdorder, stream_));
+                launches_++;
+            }
+        }
+        PSV_HIP(hipEventRecord(ev1_, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        float ms = 0;
+        PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        last_ms_ = ms;
+        adj_grad_done_ = false;
+        disc_done_ = true;
+        return PSVFWI_OK;
+    }
+
+    // device pointers into one saved state (state_copy layout)
+    struct View {
+        const T* f[5];
+        const T* ms[4];
+        const T* mv[4];
+    };
+
+    View entry(int e) const
+    {
+        const T* base = seg_.p + (size_t)e * state_elems();
+        View v{};
+        size_t off = 0;
+        for (int f = 0; f < 5; f++) {
+            v.f[f] = base + off + org_;
+            off += (size_t)nslots_ * fst_;
+        }
+        if (g_.npml > 0) {
+            const size_t st[4] = {xst_, xst_, zst_, zst_};
+            for (int m = 0; m < 4; m++) {
+                v.ms[m] = base + off;
+                off += (size_t)nslots_ * st[m];
+            }
+            for (int m = 0; m < 4; m++) {
+                v.mv[m] = base + off;
+                off += (size_t)nslots_ * st[m];
+            }
+        }
+        return v;
+    }
+
+    psv::DiscreteParams<T> dparams(int it, const View& pre, const View& post)
+    {
+        psv::DiscreteParams<T> d{};
+        d.nzt = g_.nzt;
+        d.nxt = g_.nxt;
+        d.pitch = pitch_;
+        d.ppad = ppad_;
+        d.npml = g_.npml;
+        d.tp = tp_;
+        d.fsurf = g_.fsurf;
+        d.pnx = pnx_;
+        d.pnz = pnz_;
+        d.nslots = active(it);
+        d.it = it;
+        d.dt = (T)g_.dt;
+        d.dxi = (T)1.0 / (T)g_.dx;
+        d.dzi = (T)1.0 / (T)g_.dz;
+        d.hc1 = (T)g_.hc[1];
+        d.hc2 = (T)g_.hc[2];
+        d.lam = at(lam_);
+        d.mu = at(mu_);
+        d.mu_zx = at(mu_zx_);
+        d.rho_zp = at(rho_zp_);
+        d.rho_xp = at(rho_xp_);
+        d.pml = pml_.p;
+        d.fst = fst_;
+        d.xst = xst_;
+        d.zst = zst_;
+        d.pre_vx = pre.f[kF_VX];
+        d.pre_vz = pre.f[kF_VZ];
+        d.pre_mvxx = pre.ms[0];
+        d.pre_mvzx = pre.ms[1];
+        d.pre_mvxz = pre.ms[2];
+        d.pre_mvzz = pre.ms[3];
+        d.pre_msxxx = pre.mv[0];
+        d.pre_mszxx = pre.mv[1];
+        d.pre_mszzz = pre.mv[2];
+        d.pre_mszxz = pre.mv[3];
+        d.post_sxx = post.f[kF_SXX];
+        d.post_szx = post.f[kF_SZX];
+        d.post_szz = post.f[kF_SZZ];
+        d.vbx = at(adj_.f[kF_VX]);
+        d.vbz = at(adj_.f[kF_VZ]);
+        d.sbxx = at(adj_.f[kF_SXX]);
+        d.sbzx = at(adj_.f[kF_SZX]);
+        d.sbzz = at(adj_.f[kF_SZZ]);
+        d.bvxx = adj_.ms[0].p;
+        d.bvzx = adj_.ms[1].p;
+        d.bvxz = adj_.ms[2].p;
+        d.bvzz = adj_.ms[3].p;
+        d.bsxxx = adj_.mv[0].p;
+        d.bszxx = adj_.mv[1].p;
+        d.bszzz = adj_.mv[2].p;
+        d.bszxz = adj_.mv[3].p;
+        d.w0 = at(dw_[0]);
+        d.w1 = at(dw_[1]);
+        d.w2 = at(dw_[2]);
+        d.w3 = at(dw_[3]);
+        d.g_lam = at(dg_[0]);
+        d.g_mu = at(dg_[1]);
+        d.g_mu_zx = at(dg_[2]);
+        d.g_rho_zp = at(dg_[3]);
+        d.g_rho_xp = at(dg_[4]);
+        d.grp = adj_tab_.inj_grp.p;
+        d.grp_off = dgrp_off_.p;
+        d.grp_row = adj_tab_.inj_row.p;
+        d.sig = adj_sig_.p;
+        d.sig_off = rec_out_off_.p;
+        d.shot_nt = shot_nt_.p;
+        return d;
+    }
+
     // ------------------------------------------------------------------ getters
+    psvfwi_status get_coefficient_gradient(int shot, double* gl, double* gm, double* gmzx, double* grzp,
+                                           double* grxp) const override
+    {
+        if (!disc_done_)
+            return fail(PSVFWI_E_STATE, "get_coefficient_gradient: run adjoint with PSVFWI_GRADIENT in the "
+                                        "discrete gradient mode first");
+        if (shot < -1 || shot >= nslots_ || !gl || !gm || !gmzx || !grzp || !grxp)
+            return fail(PSVFWI_E_ARG, "get_coefficient_gradient: bad shot or null array");
+        const int nzt = g_.nzt, nxt = g_.nxt;
+        double* out[5] = {gl, gm, gmzx, grzp, grxp};
+        std::vector<double> h(fst_);
+        for (int a = 0; a < 5; a++) {
+            const bool avg = a >= 2;
+            const int rows = avg ? nzt - 1 : nzt, cols = avg ? nxt - 1 : nxt;
+            std::fill(out[a], out[a] + (size_t)rows * cols, 0.0);
+            // FP64 sum over the shots in shot order
+            const int s0 = shot < 0 ? 0 : shot, s1 = shot < 0 ? nslots_ : shot + 1;
+            for (int i = s0; i < s1; i++) {
+                const int sl = slot_of_shot_[(size_t)i];
+                PSV_TRY(download(dg_[a].p + (size_t)sl * fst_, fst_, h.data()));
+                for (int iz = 0; iz < rows; iz++)
+                    for (int ix = 0; ix < cols; ix++)
+                        out[a][(size_t)iz * cols + ix] += h[(size_t)org_ + (size_t)iz * pitch_ + ix];
+            }
+        }
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status get_model_gradient(int shot, double* gl, double* gm, double* gr) const override
+    {
+        if (!gl || !gm || !gr) return fail(PSVFWI_E_ARG, "get_model_gradient: null array");
+        const size_t n = (size_t)g_.nzt * g_.nxt, na = (size_t)(g_.nzt - 1) * (g_.nxt - 1);
+        std::vector<double> c0(n), c1(n), c2(na), c3(na), c4(na);
+        PSV_TRY(get_coefficient_gradient(shot, c0.data(), c1.data(), c2.data(), c3.data(), c4.data()));
+        return psvfwi_chain_gradient(g_.nzt, g_.nxt, h_mu_.data(), h_mu_zx_.data(), h_rho_zp_.data(),
+                                     h_rho_xp_.data(), c0.data(), c1.data(), c2.data(), c3.data(), c4.data(),
+                                     gl, gm, gr);
+    }
+
     psvfwi_status get_records(int shot, double* out) const override
     {
         if (shot < 0 || shot >= nslots_ || !out) return fail(PSVFWI_E_ARG, "get_records: bad shot");
@@ -1079,19 +1328,27 @@ private:
         return (5 * fst_ + m) * sizeof(T);
     }
 
-    psvfwi_status alloc_checkpoints()
+    psvfwi_status alloc_checkpoints(int k)
     {
-        ncp_ = (ntmax_ + ck_k_ - 1) / ck_k_;
-        PSV_HIP(ckpt_.alloc((size_t)ncp_ * nslots_ * (ck_bytes_per_slot() / sizeof(T))));
+        ncp_ = (ntmax_ + k - 1) / k;
+        PSV_HIP(ckpt_.alloc((size_t)ncp_ * state_elems()));
         return PSVFWI_OK;
     }
 
-    // copy/restore the first na slots: each array is [nslots][stride], so a
-    // prefix of slots is one contiguous range
+    // elements of one saved state of every slot (checkpoint or segment entry)
+    size_t state_elems() const { return (size_t)nslots_ * (ck_bytes_per_slot() / sizeof(T)); }
+
     template <bool SAVE>
     psvfwi_status ck_copy(int c, int na, State<T>& st)
     {
-        T* base = ckpt_.p + (size_t)c * nslots_ * (ck_bytes_per_slot() / sizeof(T));
+        return state_copy<SAVE>(ckpt_.p + (size_t)c * state_elems(), na, st);
+    }
+
+    // copy/restore the first na slots of a saved state at base: each array is
+    // [nslots][stride], so a prefix of slots is one contiguous range
+    template <bool SAVE>
+    psvfwi_status state_copy(T* base, int na, State<T>& st)
+    {
         size_t off = 0;
         auto one = [&](T* arr, size_t stride) -> psvfwi_status {
             T* ck = base + off;
@@ -1352,6 +1609,15 @@ private:
     int nsnap_slot(int s) const { return snap_interval_ > 0 ? (nt_slot_[(size_t)s] - 1) / snap_interval_ + 1 : 0; }
     std::vector<int> fwd_grp_comp_;
     bool fwd_done_ = false, fwd_grad_done_ = false, store_loaded_ = false, adj_grad_done_ = false;
+    // discrete gradient: mode, segment length, the forward's checkpoint
+    // interval, adjoint state, derivative-term weights, coefficient gradients
+    // (lam, mu, mu_zx, rho_zp, rho_xp), segment states, host model copies
+    int grad_mode_ = PSVFWI_GRADIENT_REFERENCE, disc_k_ = 64, ck_used_k_ = 200;
+    bool fwd_disc_ = false, disc_done_ = false;
+    State<T> adj_;
+    DBuf<T> dw_[4], dg_[5], seg_;
+    DBuf<int> dgrp_off_;
+    std::vector<double> h_pml_, h_mu_, h_mu_zx_, h_rho_zp_, h_rho_xp_;
     bool have_obs_ = false;
     double last_ms_ = 0;
     long launches_ = 0;
@@ -1551,6 +1817,62 @@ psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* gra
 {
     PSV_PLAN(plan);
     return plan->get_gradient(shot, grad_lam, grad_mu, grad_rho);
+}
+
+psvfwi_status psvfwi_set_gradient_mode(psvfwi_plan* plan, int mode, int segment)
+{
+    PSV_PLAN(plan);
+    return plan->set_gradient_mode(mode, segment);
+}
+
+psvfwi_status psvfwi_get_coefficient_gradient(const psvfwi_plan* plan, int shot, double* g_lam, double* g_mu,
+                                              double* g_mu_zx, double* g_rho_zp, double* g_rho_xp)
+{
+    PSV_PLAN(plan);
+    return plan->get_coefficient_gradient(shot, g_lam, g_mu, g_mu_zx, g_rho_zp, g_rho_xp);
+}
+
+psvfwi_status psvfwi_get_model_gradient(const psvfwi_plan* plan, int shot, double* g_lam, double* g_mu,
+                                        double* g_rho)
+{
+    PSV_PLAN(plan);
+    return plan->get_model_gradient(shot, g_lam, g_mu, g_rho);
+}
+
+psvfwi_status psvfwi_chain_gradient(int nzt, int nxt, const double* mu, const double* mu_zx,
+                                    const double* rho_zp, const double* rho_xp, const double* g_lam,
+                                    const double* g_mu, const double* g_mu_zx, const double* g_rho_zp,
+                                    const double* g_rho_xp, double* out_lam, double* out_mu, double* out_rho)
+{
+    if (nzt < 2 || nxt < 2) return fail(PSVFWI_E_ARG, "chain_gradient: grid smaller than 2 x 2");
+    if (!mu || !mu_zx || !rho_zp || !rho_xp || !g_lam || !g_mu || !g_mu_zx || !g_rho_zp || !g_rho_xp ||
+        !out_lam || !out_mu || !out_rho)
+        return fail(PSVFWI_E_ARG, "chain_gradient: null array");
+    // mu_zx = 4 / sum(1 / mu) over the 4 corners, rho_zp = 2 / (rho(iz) + rho(iz + 1)),
+    // rho_xp = 2 / (rho(ix) + rho(ix + 1)) (average_parameters, CUDA_week2/fd_medium.cu:128-159)
+    const int ax = nxt - 1;
+    for (int iz = 0; iz < nzt; iz++) {
+        for (int ix = 0; ix < nxt; ix++) {
+            const size_t q = (size_t)iz * nxt + ix;
+            double gm = g_mu[q], gr = 0.0;
+            // averaged cells with this cell as corner: (iz, ix), (iz, ix-1), (iz-1, ix), (iz-1, ix-1)
+            for (int dz = 0; dz <= 1; dz++) {
+                for (int dx = 0; dx <= 1; dx++) {
+                    const int oz = iz - dz, ox = ix - dx;
+                    if (oz < 0 || ox < 0 || oz >= nzt - 1 || ox >= ax) continue;
+                    const size_t o = (size_t)oz * ax + ox;
+                    const double m = mu_zx[o];
+                    if (m != 0.0) gm += g_mu_zx[o] * (m * m / (4.0 * mu[q] * mu[q]));
+                    if (dx == 0) gr += g_rho_zp[o] * (-0.5 * rho_zp[o] * rho_zp[o]);
+                    if (dz == 0) gr += g_rho_xp[o] * (-0.5 * rho_xp[o] * rho_xp[o]);
+                }
+            }
+            out_lam[q] = g_lam[q];
+            out_mu[q] = gm;
+            out_rho[q] = gr;
+        }
+    }
+    return PSVFWI_OK;
 }
 
 psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches)

@@ -34,7 +34,9 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_get_store", "psvfwi_get_gradient", "psvfwi_last_timing", "psvfwi_kernel_timing",
            "psvfwi_parabola_step", "psvfwi_holberg", "psvfwi_cpml_profiles",
            "psvfwi_average_parameters", "psvfwi_ricker", "psvfwi_set_snapshots",
-           "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps")
+           "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps",
+           "psvfwi_set_gradient_mode", "psvfwi_get_coefficient_gradient", "psvfwi_get_model_gradient",
+           "psvfwi_chain_gradient")
 
 
 class PsvfwiError(RuntimeError):
@@ -240,6 +242,8 @@ class Plan:
 
     def _fwi_dims(self, shot):
         w = self.window
+        if w is None:
+            raise PsvfwiError(-5, "no FWI window set")
         nft = 1 + (self._shots[shot]["nt"] - 1) // w["dt"]
         nfz = 1 + (w["z2"] - w["z1"]) // w["dz"]
         nfx = 1 + (w["x2"] - w["x1"]) // w["dx"]
@@ -256,6 +260,30 @@ class Plan:
         g = [np.zeros((nfz, nfx)) for _ in range(3)]
         _check(lib().psvfwi_get_gradient(self._h, int(shot), *[_dptr(a) for a in g]))
         return g  # lam, mu, rho
+
+    def set_gradient_mode(self, mode, segment=64):
+        """'reference' (the correlation gradient of adjoint_kernel_PSV) or
+        'discrete' (the exact gradient of the misfit of this scheme, with
+        checkpoints every `segment` steps)."""
+        m = {"reference": 0, "discrete": 1}[mode] if isinstance(mode, str) else int(mode)
+        _check(lib().psvfwi_set_gradient_mode(self._h, m, int(segment)))
+
+    def coefficient_gradient(self, shot=-1):
+        """Discrete mode: dJ/d lam, mu [nzt, nxt] and dJ/d mu_zx, rho_zp,
+        rho_xp [nzt-1, nxt-1] (dict)."""
+        nzt, nxt = self.grid["nzt"], self.grid["nxt"]
+        g = dict(g_lam=np.zeros((nzt, nxt)), g_mu=np.zeros((nzt, nxt)),
+                 g_mu_zx=np.zeros((nzt - 1, nxt - 1)), g_rho_zp=np.zeros((nzt - 1, nxt - 1)),
+                 g_rho_xp=np.zeros((nzt - 1, nxt - 1)))
+        _check(lib().psvfwi_get_coefficient_gradient(self._h, int(shot), *[_dptr(a) for a in g.values()]))
+        return g
+
+    def model_gradient(self, shot=-1):
+        """Discrete mode: dJ/d lam, dJ/d mu, dJ/d rho [nzt, nxt]."""
+        nzt, nxt = self.grid["nzt"], self.grid["nxt"]
+        g = [np.zeros((nzt, nxt)) for _ in range(3)]
+        _check(lib().psvfwi_get_model_gradient(self._h, int(shot), *[_dptr(a) for a in g]))
+        return g
 
     def timing(self):
         ms, n = C.c_double(), C.c_long()
@@ -302,6 +330,20 @@ def average_parameters(mu, rho):
     _check(lib().psvfwi_average_parameters(int(nzt), int(nxt), _dptr(mu), _dptr(rho),
                                            *[_dptr(o) for o in out]))
     return tuple(out)  # mu_zx, rho_zp, rho_xp
+
+
+def chain_gradient(mu, mu_zx, rho_zp, rho_xp, g):
+    """lam / mu / rho gradients from coefficient gradients g (dict as
+    returned by Plan.coefficient_gradient), on the host."""
+    a = lambda x: np.ascontiguousarray(x, np.float64)
+    mu = a(mu)
+    nzt, nxt = mu.shape
+    out = [np.zeros((nzt, nxt)) for _ in range(3)]
+    arrs = [mu, a(mu_zx), a(rho_zp), a(rho_xp)] + [a(g[k]) for k in ("g_lam", "g_mu", "g_mu_zx",
+                                                                       "g_rho_zp", "g_rho_xp")]
+    _check(lib().psvfwi_chain_gradient(int(nzt), int(nxt), *[_dptr(x) for x in arrs],
+                                       *[_dptr(o) for o in out]))
+    return out
 
 
 def ricker(nt, dt, amp, fc, ts=0.0):

@@ -176,6 +176,41 @@ psvfwi_status psvfwi_get_store(const psvfwi_plan* plan, int shot, double* out); 
 psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* grad_lam,
                                   double* grad_mu, double* grad_rho);   /* [nfz][nfx] each */
 
+/* ---- discrete gradient (no counterpart in the reference) ----
+ * The reference's gradient (adjoint_kernel_PSV_GPU.cu:830-863) correlates a
+ * time-reversed wavefield with stored forward fields; it is not the
+ * derivative of the misfit of the discrete scheme.  In the discrete mode,
+ * psvfwi_forward(PSVFWI_GRADIENT) keeps checkpoints of the full state every
+ * `segment` steps, and psvfwi_adjoint(PSVFWI_GRADIENT) runs the exact
+ * transpose of the scheme (CPML, free surface and mirror included), which
+ * gives the gradient of J = sum 0.5 dt (d - d_obs)^2 (the misfit of
+ * psvfwi_misfit) with respect to the five coefficient arrays.  The adjoint
+ * source of receiver r at step t is dt * residual[r][t] (residuals from
+ * psvfwi_misfit, or signals from psvfwi_set_adjoint_sources).  No FWI window
+ * is used: the gradients cover the whole grid.  Sources must be vx or vz
+ * forces (comp 1 or 2), and the CPML profile must have a = a_half = 0 at index
+ * npml (psvfwi_cpml_profiles does).  Device memory: (segment + 1) full states
+ * per shot besides the checkpoints. */
+#define PSVFWI_GRADIENT_REFERENCE 0 /* correlation gradient of the reference (default) */
+#define PSVFWI_GRADIENT_DISCRETE 1  /* exact gradient of the discrete misfit */
+psvfwi_status psvfwi_set_gradient_mode(psvfwi_plan* plan, int mode, int segment);
+/* d J / d lam, d J / d mu [nzt][nxt]; d J / d mu_zx, rho_zp, rho_xp [nzt-1][nxt-1];
+ * shot == -1: sum over the batch (FP64, shot order) */
+psvfwi_status psvfwi_get_coefficient_gradient(const psvfwi_plan* plan, int shot, double* g_lam,
+                                              double* g_mu, double* g_mu_zx, double* g_rho_zp,
+                                              double* g_rho_xp);
+/* d J / d lam, d J / d mu, d J / d rho [nzt][nxt], for averages made by
+ * psvfwi_average_parameters from the model given to psvfwi_set_model */
+psvfwi_status psvfwi_get_model_gradient(const psvfwi_plan* plan, int shot, double* g_lam,
+                                        double* g_mu, double* g_rho);
+/* The chain rule of psvfwi_get_model_gradient on the host: out_* [nzt][nxt]
+ * from the coefficient gradients and mu, mu_zx, rho_zp, rho_xp. */
+psvfwi_status psvfwi_chain_gradient(int nzt, int nxt, const double* mu, const double* mu_zx,
+                                    const double* rho_zp, const double* rho_xp, const double* g_lam,
+                                    const double* g_mu, const double* g_mu_zx, const double* g_rho_zp,
+                                    const double* g_rho_xp, double* out_lam, double* out_mu,
+                                    double* out_rho);
+
 /* Timing of the last forward/adjoint call (milliseconds, HIP events) and the
  * number of step-kernel launches it made. */
 psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches);
