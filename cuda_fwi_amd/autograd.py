@@ -2,20 +2,26 @@
 
 ``PSVMisfit.apply(lam, mu, rho, problem)`` returns the FP64 misfit
 sum_shots sum_rec sum_t 0.5 dt (d - d_obs)^2 of a batch of shots for the
-Lame parameters and density on the padded grid.  Its backward runs the
-adjoint propagation of the same plan and returns the NEGATED lambda / mu /
-rho gradients that adjoint_kernel_PSV accumulates
-(CUDA_week2/adjoint_kernel_PSV_GPU.cu:830-863), scattered from the FWI window
-onto the grid (zero outside the window), times the incoming gradient.
+Lame parameters and density on the padded grid (the averaged mu_zx, rho_zp,
+rho_xp are formed as average_parameters does, CUDA_week2/fd_medium.cu:
+128-159).  Its backward depends on ``problem.gradient``:
 
-The reference's gradient is a correlation of the adjoint and forward
-wavefields, not the exact derivative of the discrete misfit, and it points
-against the misfit's derivative (the reference updates the model with
-m + s g, CUDA_week2/fd_medium.cu:161-240).  Negated, it is a descent-ready
-stand-in for dJ/dm: on a 64 x 64 random-smooth model its correlation with
-central finite differences of the misfit is about 0.56 (lambda), 0.60 (mu)
-and 0.98 (rho), with an arbitrary scale.  The host code (C++) stays the
-product; this module only moves tensors to and from the C ABI.
+- "discrete": the exact derivative of that misfit (the plan's discrete
+  adjoint, an extension), dJ/dlam, dJ/dmu, dJ/drho on the whole grid, times
+  the incoming gradient.  Central finite differences of the misfit agree with
+  it to rounding.
+- "reference": the NEGATED lambda / mu / rho gradients that
+  adjoint_kernel_PSV accumulates (CUDA_week2/adjoint_kernel_PSV_GPU.cu:
+  830-863), scattered from the FWI window onto the grid (zero outside it).
+  That gradient correlates the adjoint and forward wavefields; it is not the
+  derivative of the discrete misfit, and it points against it (the reference
+  updates the model with m + s g, CUDA_week2/fd_medium.cu:161-240).  Negated,
+  it is a descent direction: on a 64 x 64 random-smooth model its
+  correlation with central differences of the misfit is about 0.56 (lambda),
+  0.60 (mu) and 0.98 (rho), with an arbitrary scale.
+
+The host code (C++) stays the product; this module only moves tensors to
+and from the C ABI.
 """
 from __future__ import annotations
 
@@ -47,6 +53,8 @@ class Problem:
     window: dict
     precision: str = "fp64"
     tile: int = 0
+    gradient: str = "reference"   # or "discrete"
+    segment: int = 64             # discrete: steps between checkpoints
     plan: object = field(default=None, repr=False)
 
     def get_plan(self):
@@ -56,6 +64,7 @@ class Problem:
             p.set_pml(self.pml)
             p.set_window(self.window)
             p.set_shots(self.shots)
+            p.set_gradient_mode(self.gradient, self.segment)
             if self.tile:
                 p.set_tile(self.tile)
             self.plan = p
@@ -83,6 +92,9 @@ class PSVMisfit(torch.autograd.Function):
         problem = ctx.problem
         plan = problem.get_plan()
         plan.adjoint(energy=False, gradient=True)
+        if problem.gradient == "discrete":
+            return tuple(torch.from_numpy(g).to(ctx.like.device, ctx.like.dtype) * grad_out
+                         for g in plan.model_gradient(-1)) + (None,)
         gl, gm, gr = plan.gradient(-1)
         w = problem.window
         full = []

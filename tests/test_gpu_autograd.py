@@ -8,7 +8,9 @@ observed data from the model and the gradient taken at its smoothed version.
   50 seeded grid points (seed 4), the backward gradient has the direction of
   dJ/dm: correlation above 0.45 (lambda), 0.45 (mu) and 0.95 (rho); the
   reference's gradient is not the exact discrete derivative (its magnitude
-  differs by an arbitrary factor), which is why the check is on direction.
+  differs by an arbitrary factor), which is why the check is on direction;
+- with gradient="discrete" the backward is dJ/dm itself: the same central
+  differences agree with it to 1e-6 of max |g| at every point (FP64).
 """
 import numpy as np
 import pytest
@@ -24,7 +26,7 @@ from gpucase import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _case(precision="fp64"):
+def _case(precision="fp64", gradient="reference"):
     w = WL.c3()
     s = w.shots[0]
     sig = O.create_wavelet(w.nt, w.dt, 1.0, w.fc, 0.0, 1)[None]
@@ -41,7 +43,7 @@ def _case(precision="fp64"):
     l0, m0, r0 = w.padded(vp0, vp0 / 1.8, 310.0 * vp0 ** 0.25)
     pb = Problem(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
                  dz=w.dx, hc=O.holbergcoeff(w.fdorder, 1), pml=pml, shots=[ps], observed=[obs], window=win,
-                 precision=precision)
+                 precision=precision, gradient=gradient)
     return w, true, pb, (l0, m0, r0), win
 
 
@@ -110,3 +112,44 @@ def test_autograd_direction_against_central_differences():
         print(f"  {('lambda', 'mu', 'rho')[k]}: correlation with central differences {c:.3f}, "
               f"scale fd/grad {np.dot(fd, gg) / np.dot(gg, gg):.3e}")
         assert c > floor
+
+
+def test_autograd_discrete_gradient_equals_central_differences():
+    """The 64 x 64 differentiation case with gradient="discrete": central
+    differences (step 1e-4 relative) at 50 seeded points (seed 4) for lambda,
+    mu and rho against the backward gradient, next to the reference
+    gradient's direction on the same points."""
+    print("autograd discrete vs central differences", flush=True)
+    w, true, pb, model, win = _case("fp64", "discrete")
+    J, gl, gm, gr = _grads(pb, model)
+    _, _, pbr, _, _ = _case("fp64")
+    _, rl, rm, rr = _grads(pbr, model)
+    rng = np.random.default_rng(4)
+    pts = [(int(rng.integers(w.zoff + 2, w.zoff + w.nz - 2)), int(rng.integers(w.ppad + 2, w.ppad + w.nx - 2)))
+           for _ in range(50)]
+    plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
+                  dz=w.dx, hc=pb.hc, precision="fp64")
+    plan.set_pml(pb.pml)
+    plan.set_shots(pb.shots)
+
+    def misfit(arrs):
+        plan.set_model(arrs[0], arrs[1], *P.average_parameters(arrs[1], arrs[2]))
+        plan.forward(energy=False)
+        return float(plan.misfit(pb.observed)[0])
+
+    for k, g, gref in ((0, gl, rl), (1, gm, rm), (2, gr, rr)):
+        fd, gg, gr_ = [], [], []
+        for z, x in pts:
+            h = 1e-4 * model[k][z, x]
+            up = [a.copy() for a in model]
+            dn = [a.copy() for a in model]
+            up[k][z, x] += h
+            dn[k][z, x] -= h
+            fd.append((misfit(up) - misfit(dn)) / (2 * h))
+            gg.append(g[z, x])
+            gr_.append(gref[z, x])
+        fd, gg = np.array(fd), np.array(gg)
+        err = np.abs(fd - gg).max() / np.abs(g).max()
+        print(f"  {('lambda', 'mu', 'rho')[k]}: max |fd - g| / max |g| = {err:.2e}; correlation "
+              f"discrete {np.corrcoef(fd, gg)[0, 1]:.6f}, reference {np.corrcoef(fd, gr_)[0, 1]:.3f}")
+        assert err < 1e-6
