@@ -511,6 +511,8 @@ void MI355XOptions::parse(const char* path)
     tile = (int)r.GetInteger(s, "tile", tile);
     observed = r.Get(s, "observed", observed);
     gradient_scaling = r.Get(s, "gradient_scaling", gradient_scaling);
+    gradient = r.Get(s, "gradient", gradient);
+    gradient_segment = (int)r.GetInteger(s, "gradient_segment", gradient_segment);
     output_dir = r.Get(s, "output_dir", output_dir);
     write_csv = r.GetBoolean(s, "write_csv", write_csv);
     if (precision != "fp32" && precision != "fp64") throw std::runtime_error("[mi355x] precision: fp32 or fp64");
@@ -521,4 +523,7 @@ void MI355XOptions::parse(const char* path)
     if (!observed.empty() && observed != "modified" && observed != "file")
         throw std::runtime_error("[mi355x] observed: empty, modified or file");
     if (checkpoint_interval < 1) throw std::runtime_error("[mi355x] checkpoint_interval must be >= 1");
+    if (gradient != "reference" && gradient != "discrete")
+        throw std::runtime_error("[mi355x] gradient: reference or discrete");
+    if (gradient_segment < 1) throw std::runtime_error("[mi355x] gradient_segment must be >= 1");
 }

@@ -14,6 +14,7 @@
 // compares the first trial's misfit of the last shot against the current
 // shot's misfit; here one update per iteration uses the summed misfit and
 // the sum of the shots' scaled gradients (identical for one shot).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <filesystem>
@@ -239,6 +240,8 @@ static void build_plan(FDWaveModel& m, FDWaveModel::Impl* impl, bool with_window
         check(psvfwi_set_window(p, &w), "set_window");
         if (m.dev.gradient_store == "checkpoint")
             check(psvfwi_set_store_mode(p, PSVFWI_STORE_CHECKPOINT, m.dev.checkpoint_interval), "set_store_mode");
+        if (m.dev.gradient == "discrete")
+            check(psvfwi_set_gradient_mode(p, PSVFWI_GRADIENT_DISCRETE, m.dev.gradient_segment), "set_gradient_mode");
     }
     const int nt = m.grid.nt;
     impl->src_idx.clear();
@@ -394,7 +397,7 @@ static double batch_misfit(FDWaveModel& m, FDWaveModel::Impl* impl, int flags)
 real_sim FDWaveModel::fwi_step_length_estimation(real_sim est, real_sim* L2_test)
 {
     const int nfz = 1 + (fwi.z2 - fwi.z1) / fwi.dz, nfx = 1 + (fwi.x2 - fwi.x1) / fwi.dx;
-    const bool clean = dev.gradient_scaling == "clean";
+    const bool clean = dev.gradient_scaling == "clean" || dev.gradient == "discrete";
     real_sim step_length[4] = {0, 0, 0, 0};
     const double rho_factor = 0.5, grow = 2.0;
     const unsigned stepmax = 6;
@@ -448,12 +451,46 @@ real_sim FDWaveModel::fwi_step_length_estimation(real_sim est, real_sim* L2_test
     return calc_opt_step(L2_test, step_length);
 }
 
+// Discrete gradient (an extension): the descent direction on the FWI window
+// is -dJ/dm of the batch, divided by the source illumination sum_shots Ws
+// plus 0.005 of its maximum over the window (a positive diagonal
+// preconditioner, in place of the reference's energy scaling).
+static void discrete_direction(const FDWaveModel& m, psvfwi_plan* p, const std::vector<double>& Ws, int nfz,
+                               int nfx)
+{
+    const int nzt = m.grid.nzt, nxt = m.grid.nxt;
+    const size_t plane = (size_t)nzt * nxt;
+    std::vector<double> g[3] = {std::vector<double>(plane), std::vector<double>(plane), std::vector<double>(plane)};
+    check(psvfwi_get_model_gradient(p, -1, g[0].data(), g[1].data(), g[2].data()), "get_model_gradient");
+    std::vector<double> ill(plane, 0.0);
+    for (unsigned int s = 0; s < m.nshots; s++)
+        for (size_t i = 0; i < plane; i++) ill[i] += Ws[s * plane + i];
+    double imax = 0.0;
+    for (int zf = 0; zf < nfz; zf++)
+        for (int xf = 0; xf < nfx; xf++) {
+            const int iz = m.fwi.z1 + zf * m.fwi.dz, ix = m.fwi.x1 + xf * m.fwi.dx;
+            if (iz < nzt && ix < nxt) imax = std::max(imax, ill[(size_t)iz * nxt + ix]);
+        }
+    real_sim** out[3] = {m.fwi.grad_lam, m.fwi.grad_mu, m.fwi.grad_rho};
+    for (int zf = 0; zf < nfz; zf++)
+        for (int xf = 0; xf < nfx; xf++) {
+            const int iz = m.fwi.z1 + zf * m.fwi.dz, ix = m.fwi.x1 + xf * m.fwi.dx;
+            for (int k = 0; k < 3; k++) out[k][zf][xf] = 0.0;
+            if (iz >= nzt || ix >= nxt) continue;
+            const size_t q = (size_t)iz * nxt + ix;
+            const double w = ill[q] + 0.005 * imax;
+            if (w <= 0.0) continue;
+            for (int k = 0; k < 3; k++) out[k][zf][xf] = -g[k][q] / w;
+        }
+}
+
 // CUDA_week2/fd_wave_model.cu:237-968 (the CPU branch's sequence), batched
 void FDWaveModel::fwi_simulate()
 {
     const int nzt = grid.nzt, nxt = grid.nxt, nt = grid.nt;
     const int nfz = 1 + (fwi.z2 - fwi.z1) / fwi.dz, nfx = 1 + (fwi.x2 - fwi.x1) / fwi.dx;
-    const bool clean = dev.gradient_scaling == "clean";
+    const bool discrete = dev.gradient == "discrete";
+    const bool clean = dev.gradient_scaling == "clean" || discrete;
     medium.allocate_medium(nzt, nxt);
     medium.velocity_to_lame();
     medium.allocate_medium_av(nzt - 1, nxt - 1);
@@ -519,6 +556,9 @@ void FDWaveModel::fwi_simulate()
         for (unsigned int s = 0; s < nshots; s++) check(psvfwi_get_field(p, (int)s, 5, &Ws[s * plane]), "get_field");
         check(psvfwi_adjoint(p, PSVFWI_ENERGY | PSVFWI_GRADIENT), "adjoint");
 
+        if (discrete) {
+            discrete_direction(*this, p, Ws, nfz, nfx);
+        } else {
         reset_array_2d(fwi.grad_lam, nfz, nfx);
         reset_array_2d(fwi.grad_mu, nfz, nfx);
         reset_array_2d(fwi.grad_rho, nfz, nfx);
@@ -534,6 +574,7 @@ void FDWaveModel::fwi_simulate()
             }
         }
         std::memcpy(wave.We[0], Wr.data(), sizeof(double) * plane);
+        }
 
         step = fwi_step_length_estimation(step, L2_test);
         std::cout << "Optimum step length: " << step << std::endl;

@@ -160,6 +160,9 @@ struct MI355XOptions {
     std::string observed = "";           // "" (reference: zeros), "modified" (forward of the
                                          // stagger_over_grid_modify model) or "file" (receiver signal_file)
     std::string gradient_scaling = "reference";  // reference | clean (energy weights, model update)
+    std::string gradient = "reference";  // reference (correlation) | discrete (exact misfit gradient;
+                                         // implies the clean, window-correct update)
+    int gradient_segment = 64;           // discrete: steps between checkpoints
     std::string output_dir = "./io";
     bool write_csv = true;
     void parse(const char* path);
