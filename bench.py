@@ -257,6 +257,58 @@ def run_c2(args):
     return line
 
 
+def run_c2_discrete(args):
+    """The default workload with the discrete gradient (an extension: the
+    exact misfit gradient, checkpoints every --interval steps, each segment
+    replayed and the transposed step run back through it).  Reports the
+    throughput of forward + misfit + discrete adjoint and the split between
+    forward and adjoint."""
+    from cuda_fwi_amd import workloads as WL
+    w = WL.c2()
+    if args.nt:
+        w.nt = args.nt
+        w.shots[0]["nt"] = args.nt
+    plan = make_plan(w, args.precision, None, "full", args.interval, args.tile)
+    plan.set_gradient_mode("discrete", args.interval)
+    _log("setup done (observed data computed)")
+
+    def one_pass():
+        plan.forward(energy=True, gradient=True)
+        f = plan.timing()[0]
+        plan.misfit()
+        plan.adjoint(energy=False, gradient=True)
+        return f, plan.timing()[0]
+
+    for i in range(args.warmup):
+        one_pass()
+        _log(f"warmup pass {i} done")
+    _sync()
+    t0 = time.perf_counter()
+    tf = ta = 0.0
+    for i in range(args.steps):
+        f, a = one_pass()
+        _log(f"timed pass {i}: forward {f:.1f} ms, discrete adjoint {a:.1f} ms")
+        tf += f
+        ta += a
+    _sync()
+    T = time.perf_counter() - t0
+    cells = w.interior_cells()
+    esize = 4 if args.precision == "fp32" else 8
+    return {
+        "metric": METRIC, "value": round(2.0 * cells * w.nt * args.steps / T / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(T / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if esize == 4 else "f64",
+        "data": "synthetic: seeded 8-layer model with 3% Gaussian perturbation (observed) and "
+                "the unperturbed layers (synthetic)",
+        "config": {"workload": f"1000x500 dx5 order-4 npml40, nt {w.nt}, dt 0.4 ms, 1 shot, 400 receivers, "
+                               "fwd+adj+grad with the discrete (exact) gradient on the whole grid",
+                   "checkpoint_interval": args.interval, "tile": [64, abs(plan.tile)]},
+        "forward_ms_per_step": round(tf / args.steps, 2),
+        "adjoint_ms_per_step": round(ta / args.steps, 2),
+    }
+
+
 def _groups_for_store(w, window_cells, fwi_dt, esize, budget_bytes):
     """Consecutive groups of shots (sorted by decreasing nt) whose full
     decimated stores fit in budget_bytes together."""
@@ -475,6 +527,8 @@ def main():
     ap.add_argument("--per-shot-sample", type=int, default=8)
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="replay the time loop as captured HIP graphs of this many steps (0: direct launches)")
+    ap.add_argument("--gradient", default="reference", choices=["reference", "discrete"],
+                    help="default workload: the reference's correlation gradient or the discrete (exact) one")
     ap.add_argument("--store-budget-gb", type=float, default=200.0,
                     help="batched workload with --store full: store memory per group of shots")
     args = ap.parse_args()
@@ -489,6 +543,8 @@ def main():
         print(json.dumps(calibrate_copy()), flush=True)
         return
     run = {"c2": run_c2, "c4": run_c4, "c3mem": run_c3mem}[args.workload]
+    if args.workload == "c2" and args.gradient == "discrete":
+        run = run_c2_discrete
     line = run(args)
     print(json.dumps(line), flush=True)
 

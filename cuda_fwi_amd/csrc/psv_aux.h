@@ -18,4 +18,16 @@ hipError_t launch_grad_sum(const T* grad, size_t gst, const int* order, int nsho
 template <typename T>
 hipError_t launch_fill(T* p, size_t n, T v, hipStream_t s);
 
+// Up to kMaxCopies device-to-device copies in one launch (a state save or
+// restore: 5 wavefields and 8 CPML arrays).  Every size and address must be
+// a multiple of 16 bytes.
+constexpr int kMaxCopies = 16;
+struct CopyList {
+    const void* src[kMaxCopies];
+    void* dst[kMaxCopies];
+    size_t bytes[kMaxCopies];
+    int n;
+};
+hipError_t launch_copies(const CopyList& l, hipStream_t s);
+
 }  // namespace psv

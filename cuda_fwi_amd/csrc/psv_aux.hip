@@ -100,6 +100,33 @@ hipError_t launch_fill(T* p, size_t n, T v, hipStream_t s)
     return hipGetLastError();
 }
 
+// blockIdx.y picks the copy; the blocks of x stride over it in 16-byte units
+__global__ __launch_bounds__(256) void copies_kernel(const CopyList l)
+{
+    const int c = blockIdx.y;
+    const uint4* src = (const uint4*)l.src[c];
+    uint4* dst = (uint4*)l.dst[c];
+    const size_t n = l.bytes[c] / 16;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copies(const CopyList& l, hipStream_t s)
+{
+    if (l.n <= 0) return hipSuccess;
+    if (l.n > kMaxCopies) return hipErrorInvalidValue;
+    size_t most = 0;
+    for (int i = 0; i < l.n; i++) {
+        if ((l.bytes[i] | (size_t)l.src[i] | (size_t)l.dst[i]) & 15) return hipErrorInvalidValue;
+        if (l.bytes[i] > most) most = l.bytes[i];
+    }
+    size_t blocks = (most / 16 + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(copies_kernel, dim3((unsigned)blocks, (unsigned)l.n), dim3(256), 0, s, l);
+    return hipGetLastError();
+}
+
 template hipError_t launch_misfit<float>(const float*, const double*, float*, const size_t*,
                                          const size_t*, int, int, double, double*, double*, int*,
                                          hipStream_t);

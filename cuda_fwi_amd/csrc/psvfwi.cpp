@@ -1349,26 +1349,29 @@ private:
     template <bool SAVE>
     psvfwi_status state_copy(T* base, int na, State<T>& st)
     {
+        // one launch for all 13 arrays (sizes and offsets are multiples of 32 elements)
+        psv::CopyList l{};
         size_t off = 0;
-        auto one = [&](T* arr, size_t stride) -> psvfwi_status {
+        auto one = [&](T* arr, size_t stride) {
             T* ck = base + off;
-            const size_t bytes = (size_t)na * stride * sizeof(T);
-            if (SAVE) PSV_HIP(hipMemcpyAsync(ck, arr, bytes, hipMemcpyDeviceToDevice, stream_));
-            else PSV_HIP(hipMemcpyAsync(arr, ck, bytes, hipMemcpyDeviceToDevice, stream_));
+            l.src[l.n] = SAVE ? (const void*)arr : (const void*)ck;
+            l.dst[l.n] = SAVE ? (void*)ck : (void*)arr;
+            l.bytes[l.n] = (size_t)na * stride * sizeof(T);
+            l.n++;
             off += (size_t)nslots_ * stride;
-            return PSVFWI_OK;
         };
-        for (int f = 0; f < 5; f++) PSV_TRY(one(st.f[f].p, fst_));
+        for (int f = 0; f < 5; f++) one(st.f[f].p, fst_);
         if (g_.npml > 0) {
-            PSV_TRY(one(st.ms[0].p, xst_));
-            PSV_TRY(one(st.ms[1].p, xst_));
-            PSV_TRY(one(st.ms[2].p, zst_));
-            PSV_TRY(one(st.ms[3].p, zst_));
-            PSV_TRY(one(st.mv[0].p, xst_));
-            PSV_TRY(one(st.mv[1].p, xst_));
-            PSV_TRY(one(st.mv[2].p, zst_));
-            PSV_TRY(one(st.mv[3].p, zst_));
+            one(st.ms[0].p, xst_);
+            one(st.ms[1].p, xst_);
+            one(st.ms[2].p, zst_);
+            one(st.ms[3].p, zst_);
+            one(st.mv[0].p, xst_);
+            one(st.mv[1].p, xst_);
+            one(st.mv[2].p, zst_);
+            one(st.mv[3].p, zst_);
         }
+        PSV_HIP(psv::launch_copies(l, stream_));
         return PSVFWI_OK;
     }
 
