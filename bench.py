@@ -106,9 +106,11 @@ def alg_bytes(w, nwin, fwi_dt, esize):
     }
 
 
-def cpu_baseline(w, window, nt_sample):
-    """The oracle (single-threaded FP64 CPU restatement of the reference path)
-    on the same grid, model and geometry, for nt_sample steps."""
+def cpu_baseline(w, window, nt_sample, threaded=False):
+    """The oracle (FP64 CPU restatement of the reference path) on the same
+    grid, model and geometry, for nt_sample steps: single-threaded (the
+    reference CPU path has no threads), or its row-parallel OpenMP build over
+    the host cores OMP_NUM_THREADS allows (same results)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import psv_oracle as O
     from cuda_fwi_amd import psvfwi as P
@@ -124,15 +126,23 @@ def cpu_baseline(w, window, nt_sample):
                    lam=lam, mu=mu, rho=rho, pml=pml, src=src, src_signal=ps["src_signal"], rec=rec,
                    fwi=window)
     obs = np.zeros((len(rec["x"]), nt_sample))
-    t0 = time.perf_counter()
-    f = O.forward(pb)
-    _, res, _ = O.l2_adjoint_sources(w.dt, f["rec"], obs)
-    O.adjoint(pb, rec, res, f["store"])
-    t = time.perf_counter() - t0
-    return dict(value=2.0 * w.interior_cells() * nt_sample / t / 1e9, unit="Gcell-updates/s", cores=1,
+    threads = 1
+    if threaded:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.set_threaded(threaded)
+    try:
+        t0 = time.perf_counter()
+        f = O.forward(pb)
+        _, res, _ = O.l2_adjoint_sources(w.dt, f["rec"], obs)
+        O.adjoint(pb, rec, res, f["store"])
+        t = time.perf_counter() - t0
+    finally:
+        O.set_threaded(False)
+    return dict(value=2.0 * w.interior_cells() * nt_sample / t / 1e9, unit="Gcell-updates/s", cores=threads,
                 kind="port",
-                sample=f"oracle FP64, 1 thread: same grid/model/geometry, forward+misfit+adjoint+"
-                       f"gradient over nt={nt_sample} of {w.nt} steps ({t:.1f} s)")
+                sample=f"oracle FP64, {threads} thread{'s' if threads > 1 else ''}: same grid/model/geometry, "
+                       f"forward+misfit+adjoint+gradient over nt={nt_sample} of {w.nt} steps ({t:.1f} s)")
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
@@ -254,6 +264,8 @@ def run_c2(args):
     if not args.no_cpu_baseline:
         _log("cpu baseline (oracle) starting")
         line["cpu_baseline"] = cpu_baseline(w, window, args.cpu_nt)
+        _log("cpu baseline (threaded oracle) starting")
+        line["cpu_baseline_threads"] = cpu_baseline(w, window, args.cpu_nt, threaded=True)
     return line
 
 

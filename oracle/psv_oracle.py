@@ -15,7 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("PSV_ORACLE_LIB") or os.path.join(_HERE, "libpsv_oracle.so")
+_OMP_PATH = os.path.join(_HERE, "libpsv_oracle_omp.so")
 _lib = None
+_threaded = False   # set_threaded(True): the row-parallel build (same results, all host cores)
 
 
 class Cfg(C.Structure):
@@ -37,15 +39,34 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    h = C.CDLL(path)
+    d = C.POINTER(C.c_double)
+    h.psvo_calc_opt_step.restype = C.c_double
+    h.psvo_calc_opt_step.argtypes = [d, d]
+    return h
+
+
+_omp = None
+
+
+def set_threaded(on: bool):
+    """Route the propagators through the OpenMP build (rows of each sweep in
+    parallel; bit-identical results) or the single-threaded one."""
+    global _threaded
+    _threaded = bool(on)
+
+
 def lib():
-    global _lib
+    global _lib, _omp
+    if _threaded:
+        if _omp is None:
+            _omp = _load(_OMP_PATH)
+        return _omp
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        _lib = C.CDLL(_LIB_PATH)
-        d = C.POINTER(C.c_double)
-        _lib.psvo_calc_opt_step.restype = C.c_double
-        _lib.psvo_calc_opt_step.argtypes = [d, d]
+        _lib = _load(_LIB_PATH)
     return _lib
 
 

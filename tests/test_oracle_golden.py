@@ -431,3 +431,27 @@ def test_discrete_adjoint_dot_product():
         return O.forward(pb)["rec"]
     rhs = np.sum(r * (traces(lam + dlam) - traces(lam - dlam)) / 2)
     assert abs(lhs - rhs) < 1e-6 * abs(rhs)
+
+
+def test_threaded_oracle_is_bit_identical():
+    """The row-parallel OpenMP build (the multi-core CPU baseline) gives the
+    single-threaded oracle's bits: forward fields, traces, store, gradients."""
+    import os
+    os.environ.setdefault("OMP_NUM_THREADS", "4")
+    pb = RC.small_problem(nx=50, nz=40, npml=8, nt=150, fdorder=4, fsurf=0, seed=2)
+    pb.fwi = RC.full_window(pb, dt=2)
+    res = np.random.default_rng(1).standard_normal((len(pb.rec["x"]), pb.nt)) * 1e-3
+    out = []
+    for threaded in (False, True):
+        O.set_threaded(threaded)
+        try:
+            f = O.forward(pb)
+            a = O.adjoint(pb, pb.rec, res, f["store"])
+        finally:
+            O.set_threaded(False)
+        out.append((f, a))
+    (f0, a0), (f1, a1) = out
+    for k in ("rec", "vx", "vz", "sxx", "szx", "szz", "We", "store", "mem"):
+        assert np.array_equal(f0[k], f1[k]), k
+    for k in ("grad_lam", "grad_mu", "grad_rho", "vx", "szz"):
+        assert np.array_equal(a0[k], a1[k]), k
