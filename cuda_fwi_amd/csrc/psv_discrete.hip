@@ -204,7 +204,8 @@ __global__ void d_mirror_kernel(const DiscreteParams<T> p)
     const size_t so = (size_t)blockIdx.y * p.fst;
     for (int kk = 1; kk <= H; kk++) {
         const size_t g = so + (long)(H - kk) * p.pitch + ix, o = so + (long)(H + kk) * p.pitch + ix;
-        p.sbzx[o] = p.sbzx[o] - p.sbzx[g];
+        const size_t ox = ORDER == 2 ? o : o - p.pitch;  // order 4: szx mirrors row H + kk - 1
+        p.sbzx[ox] = p.sbzx[ox] - p.sbzx[g];
         p.sbzz[o] = p.sbzz[o] - p.sbzz[g];
         p.sbzx[g] = (T)0;
         p.sbzz[g] = (T)0;
@@ -285,16 +286,22 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_weights_kernel(const Discre
     const T bxx = p.sbxx[k], bzx = p.sbzx[k], bzz = p.sbzz[k];
     T wxx, wzz, wzx;
     if (fsurf && iz == H) {
-        // free surface: sxx = 4 dt vx_x (l m + m^2) / (l + 2 m), szz = szx = 0
+        // free surface, order 2: sxx = 4 dt vx_x (l m + m^2) / (l + 2 m), szz = szx = 0;
+        // order 4: sxx += the same, szz = 0, szx updated as elsewhere
         const T l2m = l + (T)2.0 * m;
         const T den = l2m * l2m;
         wxx = bxx * ((T)4.0 * dt * (l * m + m * m) / l2m);
         wzz = (T)0;
-        wzx = (T)0;
         p.g_lam[k] = p.g_lam[k] + bxx * ((T)4.0 * dt * vx_x * (m * m) / den);
         p.g_mu[k] = p.g_mu[k] + bxx * ((T)4.0 * dt * vx_x * (l * l + (T)2.0 * l * m + (T)2.0 * m * m) / den);
-        p.sbxx[k] = (T)0;
-        p.sbzx[k] = (T)0;
+        if (ORDER == 2) {
+            wzx = (T)0;
+            p.sbxx[k] = (T)0;
+            p.sbzx[k] = (T)0;
+        } else {
+            wzx = dt * p.mu_zx[c] * bzx;
+            p.g_mu_zx[k] = p.g_mu_zx[k] + bzx * (dt * (vz_x + vx_z));
+        }
         p.sbzz[k] = (T)0;
     } else {
         wxx = dt * (l * (bxx + bzz) + (T)2.0 * m * bxx);

@@ -353,8 +353,12 @@ __global__ __launch_bounds__(kThreads) void psv_fused_kernel(const StepParams<T>
             szzn = s[2] + dt * (lam * (vx_x + vz_z) + ((T)2.0 * mu * vz_z));
             if (fsurf && gz == fpad) {
                 szzn = (T)0.0;
-                szxn = (T)0.0;
-                sxxn = (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+                if (ORDER == 2) {
+                    szxn = (T)0.0;
+                    sxxn = (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+                } else {
+                    sxxn = s[0] + (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+                }
             }
         }
         lsxx[r][c] = sxxn;
@@ -382,7 +386,7 @@ __global__ __launch_bounds__(kThreads) void psv_fused_kernel(const StepParams<T>
             const int gz = z0 - SH + r, gx = x0 - SH + c;
             if (gz >= 0 && gz < fpad && gx >= fpad && gx < nxt - fpad) {
                 const int mr = r + 2 * (fpad - gz);
-                lszx[r][c] = -lszx[mr][c];
+                lszx[r][c] = -lszx[ORDER == 2 ? mr : mr - 1][c];
                 lszz[r][c] = -lszz[mr][c];
             }
         };

@@ -301,8 +301,14 @@ __global__ __launch_bounds__(kThreads) void psv_stress_kernel(const StepParams<T
         T szzn = rszz[k] + dt * (lam * (vx_x + vz_z) + ((T)2.0 * mu * vz_z));
         if (fsurf && iz == fpad) {
             szzn = (T)0.0;
-            szxn = (T)0.0;
-            sxxn = (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+            if (ORDER == 2) {
+                szxn = (T)0.0;
+                sxxn = (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+            } else {
+                // order 4 (extension): sxx integrates the surface relation and
+                // szx keeps its update; the order-2 form is unstable here
+                sxxn = rsxx[k] + (T)4.0 * dt * vx_x * (lam * mu + mu * mu) / (lam + (T)2.0 * mu);
+            }
         }
         stg(sxxS, o, sxxn);
         stg(szxS, o, szxn);
@@ -385,8 +391,10 @@ __global__ __launch_bounds__(kThreads) void psv_velocity_kernel(const StepParams
         hszx[k] = hszz[k] = 0;
         if (r < SHT) {
             if (top && gz >= 0 && gz < fpad && gx >= fpad && gx < nxt - fpad) {
+                // order 4: szx (half a cell below szz) mirrors about row fpad - 1/2
                 const unsigned om = boff(2 * fpad - gz, gx);
-                hszx[k] = -ldg((const T*)szxS, om);
+                const unsigned omx = ORDER == 2 ? om : boff(2 * fpad - gz - 1, gx);
+                hszx[k] = -ldg((const T*)szxS, omx);
                 hszz[k] = -ldg((const T*)szzS, om);
             } else {
                 const unsigned o = boff(gz, gx);

@@ -8,8 +8,15 @@ at (100, 2), 101 receivers along the surface, dt 0.5 ms, nt 2000.
 - observed = forward of the anomaly model, synthetic = forward of the
   homogeneous background; one forward + misfit + adjoint + gradient over the
   model window (time decimation 10), orders 4 (explosive source) and 2 (vz
-  force, the reference's own source type): gradients FP64 bit for bit, FP32
-  within 1e-4 relative L2.
+  force, the reference's own source type): gradients FP64 bit for bit and
+  FP32 bit for bit against the FP32 oracle.  Against the FP64 gradient the
+  FP32 one is limited by FP32 itself, not by the kernels: the anomaly is
+  small (20 m), its scattered field is 1e-3 to 1e-4 of the direct wave, so
+  FP32 traces resolve the residual only to about 3e-3, and the rounding noise
+  of the direct wave correlates with the strong forward field near the source
+  and receivers.  Measured: 1.7e-2 to 3.7e-2 relative L2 with the explosive
+  source; the test bounds it at 0.1 and the direction (correlation) at
+  0.999.
 """
 import numpy as np
 import pytest
@@ -80,6 +87,15 @@ def test_c1_fwd_adj_gradient(fdorder, comp):
     p32.forward(gradient=True)
     p32.misfit([obs])
     p32.adjoint(gradient=True)
-    errs = [rel_l2(x, y) for x, y in zip(p32.gradient(0), g64)]
-    print(f"  fp32 gradients rel L2 vs fp64: {[f'{e:.2e}' for e in errs]}")
-    assert max(errs) < 1e-4
+    g32 = p32.gradient(0)
+    f32 = O.forward(pb, prec="f32")
+    res32 = (f32["rec"].astype(np.float64) - obs).astype(np.float32)
+    a32 = O.adjoint(pb, pb.rec, res32, f32["store"], prec="f32")
+    for got, k in zip(g32, ("grad_lam", "grad_mu", "grad_rho")):
+        assert np.array_equal(got, a32[k].astype(np.float64)), k
+    eres = rel_l2(res32, res)
+    errs = [rel_l2(x, y) for x, y in zip(g32, g64)]
+    corr = [float(np.corrcoef(x.ravel(), y.ravel())[0, 1]) for x, y in zip(g32, g64)]
+    print(f"  fp32 residual rel L2 {eres:.2e}; fp32 gradients vs fp64: rel L2 {[f'{e:.2e}' for e in errs]}, "
+          f"correlation {[f'{c:.6f}' for c in corr]}")
+    assert max(errs) < 0.1 and min(corr) > 0.999

@@ -455,3 +455,47 @@ def test_threaded_oracle_is_bit_identical():
         assert np.array_equal(f0[k], f1[k]), k
     for k in ("grad_lam", "grad_mu", "grad_rho", "vx", "szz"):
         assert np.array_equal(a0[k], a1[k]), k
+
+
+def test_free_surface_long_records_stay_bounded():
+    """3000 steps with a free surface: the late traces decay at orders 2 and
+    4 (the reference's order-2 surface treatment carried to order 4 grows
+    without bound; the order-4 extension integrates sxx and images szx about
+    the surface instead)."""
+    for fdorder in (2, 4):
+        pb = RC.small_problem(nx=80, nz=60, npml=20, nt=3000, fdorder=fdorder, fsurf=1, hetero=False,
+                              nsrc=1, dt=5e-4, fc=15.0)
+        r = O.forward(pb)["rec"]
+        early, late = np.abs(r[:, 300:600]).max(), np.abs(r[:, 2700:]).max()
+        assert late < 1e-2 * early, (fdorder, early, late)
+
+
+@pytest.mark.parametrize("fdorder,tol", [(2, 0.06), (4, 0.02)])
+def test_rayleigh_wave_speed(fdorder, tol):
+    """Surface vz of a vertical force at the free surface: the move-out of the
+    dominant (Rayleigh) arrival between two receivers 600 m apart is
+    c_R = 0.9194 vs for a Poisson solid (vp / vs = sqrt(3)).  The reference's
+    order-2 surface (szx forced to zero half a cell down, sxx set to its
+    increment) runs about 4.6 % slow at 17 cells per Rayleigh wavelength; the
+    order-4 surface is within 1 %."""
+    npml = 20
+    fpad = fdorder // 2
+    ppad = npml + fpad
+    nx, nz = 400, 120
+    nxt, nzt = nx + 2 * ppad, nz + ppad + fpad
+    dx, dt, fc, nt = 5.0, 4e-4, 15.0, 2600
+    vs = 2500.0 / np.sqrt(3.0)
+    lam0, mu0 = O.velocity_to_lame(2500.0, vs, 2000.0)
+    lam, mu, rho = (np.full((nzt, nxt), v) for v in (lam0, mu0, 2000.0))
+    x0, r1, r2 = ppad + 40, 100, 220
+    pb = O.Problem(nt=nt, nzt=nzt, nxt=nxt, fdorder=fdorder, npml=npml, dt=dt, dx=dx, dz=dx, fsurf=1,
+                   hc=O.holbergcoeff(fdorder, 1), lam=lam, mu=mu, rho=rho,
+                   pml=O.cpml_pro(npml, dx, fc, 2.0, 1.0, 1e-3, 2750.0, dt),
+                   src=dict(x=[x0], z=[fpad + 1], comp=[2], shot=[0]),
+                   src_signal=O.create_wavelet(nt, dt, 1.0, fc, 0.0, 1)[None],
+                   rec=dict(x=[x0 + r1, x0 + r2], z=[fpad, fpad], comp=[2, 2], shot=[0, 0]))
+    d = O.forward(pb)["rec"]
+    t1, t2 = int(np.argmax(np.abs(d[0]))), int(np.argmax(np.abs(d[1])))
+    w = 150
+    c = (r2 - r1) * dx / ((t2 - t1 + _lag(d[0][t1 - w:t1 + w], d[1][t2 - w:t2 + w])) * dt)
+    assert abs(c / (0.9194 * vs) - 1) < tol, c
