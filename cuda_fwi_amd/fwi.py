@@ -1,0 +1,175 @@
+"""Multi-shot FWI on one device plan: the host loop over the C ABI for models
+that the reference's INI cannot describe (layers, faults, bodies).
+
+It follows FDWaveModel::fwi_simulate (CUDA_week2/fd_wave_model.cu:237-968)
+with the batched, window-correct choices of the C++ driver
+(cuda_fwi_amd/csrc/host/fd_wave_model.cpp, ``[mi355x] gradient = discrete``):
+
+- per iteration one batched forward with the energy weights, the FP64 misfit
+  sum_shots sum_rec sum_t 0.5 dt (d - d_obs)^2 and the discrete adjoint;
+- descent direction -dJ/dm, divided by the summed source illumination
+  sum_shots Ws plus 0.005 of its maximum over the window; optionally zero in
+  the first `mute` rows of the window, where the sources and receivers sit
+  and the gradient peaks (a common FWI practice; 0 keeps every row);
+- the reference's bracketing step search (fwi_step_length_estimation.cu:
+  11-224: halve until the misfit drops, then grow by 1.5 until it rises, at
+  most 7 tries each) with batched trial forwards, and the vertex of the
+  parabola through the three (step, misfit) points in FP64;
+- update m + s d on the window with s = step max(m) / max|d| and the bounds
+  of fwi_grad_update (fd_medium.cu:161-240), rho with half the step; the
+  padding is refilled by edge replication of the model region.
+
+Everything that touches the wavefields runs on the GPU through the plan; this
+module only moves the model (three grids per iteration) and scalars.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import psvfwi as P
+
+BOUNDS = {"rho": (1.5, 3000.0), "lam": (0.0, 48.0e9), "mu": (0.0, 27.0e9)}
+
+
+class StepSearchFailed(RuntimeError):
+    pass
+
+
+@dataclass
+class History:
+    misfit: list = field(default_factory=list)
+    step: list = field(default_factory=list)
+    trials: list = field(default_factory=list)
+    seconds: list = field(default_factory=list)
+    model_error: list = field(default_factory=list)
+
+
+def _update(m, d, step, win, bounds):
+    """m + s d on the window cells, s = step max(m) / max|d| (window maxima)."""
+    z1, z2, x1, x2 = win
+    out = m.copy()
+    w_m, w_d = m[z1:z2, x1:x2], d[z1:z2, x1:x2]
+    dmax = np.abs(w_d).max()
+    if dmax == 0.0:
+        return out
+    s = step * w_m.max() / dmax
+    out[z1:z2, x1:x2] = np.clip(w_m + s * w_d, *bounds)
+    return out
+
+
+def _repad(a, win):
+    z1, z2, x1, x2 = win
+    inner = a[z1:z2, x1:x2]
+    return np.pad(inner, ((z1, a.shape[0] - z2), (x1, a.shape[1] - x2)), mode="edge")
+
+
+def _parabola(L2, sl):
+    try:
+        s = P.parabola_step(L2, sl)
+    except P.PsvfwiError:
+        return None
+    return s if np.isfinite(s) and s > 0 else None
+
+
+class Inversion:
+    """plan: a device plan with pml and shots set (observed traces are set
+    here); model: lam, mu, rho on the padded grid; win: (z1, z2, x1, x2), the
+    cells the inversion updates."""
+
+    def __init__(self, plan: P.Plan, observed, model, win, segment=100, step=0.01, mute=0):
+        self.plan = plan
+        self.win = win
+        self.step = step
+        self.mute = int(mute)
+        self.m = {"lam": model[0].copy(), "mu": model[1].copy(), "rho": model[2].copy()}
+        self.nshots = len(observed)
+        plan.set_observed(observed)
+        plan.set_gradient_mode("discrete", segment)
+        self.forwards = 0
+        self.adjoints = 0
+
+    def _set(self, m):
+        self.plan.set_model(m["lam"], m["mu"], *P.average_parameters(m["mu"], m["rho"]))
+
+    def _misfit(self, m, gradient=False):
+        self._set(m)
+        self.plan.forward(energy=gradient, gradient=gradient)
+        self.forwards += 1
+        return float(np.sum(self.plan.misfit()))
+
+    def _trial(self, d, est):
+        m = self.m
+        t = {k: _update(m[k], d[k], est * (0.5 if k == "rho" else 1.0), self.win, BOUNDS[k]) for k in m}
+        return {k: _repad(v, self.win) for k, v in t.items()}
+
+    def iterate(self):
+        """One iteration; returns (misfit before the update, step taken, trials)."""
+        L2 = self._misfit(self.m, gradient=True)
+        ill = sum(self.plan.field("We", i) for i in range(self.nshots))
+        self.plan.adjoint(energy=False, gradient=True)
+        self.adjoints += 1
+        g = self.plan.model_gradient(-1)
+        z1, z2, x1, x2 = self.win
+        w = ill + 0.005 * ill[z1:z2, x1:x2].max()
+        d = {k: -gk / w for k, gk in zip(("lam", "mu", "rho"), g)}
+        if self.mute > 0:
+            for v in d.values():
+                v[z1:z1 + self.mute, :] = 0.0
+        # the reference's bracketing search
+        L2t, sl = [L2, 0.0, 0.0], [0.0, 0.0, 0.0]
+        est, step1, step2, itest, count, ntrial = self.step, False, False, 1, 0, 0
+        while not (step1 and step2):
+            L2t[itest] = self._misfit(self._trial(d, est))
+            sl[itest] = est
+            ntrial += 1
+            if step1 and L2t[1] < L2t[2]:
+                sl[2], step2 = est, True
+            elif step1:
+                sl[2] = est
+                est += est / 2.0
+                count += 1
+            elif L2t[0] > L2t[1]:
+                sl[1], step1, itest, count = est, True, 2, 0
+                est += est / 2.0
+            else:
+                est /= 2.0
+                count += 1
+            if not step1 and count > 6:
+                raise StepSearchFailed("Step length estimation failed")
+            if step1 and count > 6:
+                step2 = True
+        opt = _parabola(L2t, sl)
+        if opt is None:   # a parabola open downwards: keep the best tried step
+            opt = sl[int(np.argmin(L2t[1:])) + 1]
+        self.m = self._trial(d, opt)
+        self.step = opt
+        return L2, opt, ntrial
+
+    def run(self, iters, truth=None, log=None):
+        h = History()
+        for i in range(iters):
+            t0 = time.perf_counter()
+            L2, s, n = self.iterate()
+            h.seconds.append(time.perf_counter() - t0)
+            h.misfit.append(L2)
+            h.step.append(s)
+            h.trials.append(n)
+            if truth is not None:
+                h.model_error.append(self.model_error(truth))
+            if log:
+                log(f"iteration {i}: misfit {L2:.6e}, step {s:.4g}, {n} trials, {h.seconds[-1]:.1f} s"
+                    + (f", vp error {h.model_error[-1]:.4f}" if truth is not None else ""))
+        return h
+
+    def vp(self, m=None):
+        m = self.m if m is None else m
+        return np.sqrt((m["lam"] + 2.0 * m["mu"]) / m["rho"])
+
+    def model_error(self, truth):
+        """Relative L2 error of vp over the window against truth (lam, mu, rho)."""
+        z1, z2, x1, x2 = self.win
+        vt = np.sqrt((truth[0] + 2.0 * truth[1]) / truth[2])[z1:z2, x1:x2]
+        return float(np.linalg.norm(self.vp()[z1:z2, x1:x2] - vt) / np.linalg.norm(vt))

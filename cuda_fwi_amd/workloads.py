@@ -196,8 +196,11 @@ def c4(nshots=64, seed_geom=6):
     return w
 
 
-def c5(nshots=128, nrec=600, nt=6000):
-    nx, nz, dx = 2400, 800, 5.0
+def c5(nshots=128, nrec=600, nt=6000, nx=2400, nz=800):
+    """nx, nz below 2400 x 800 give the same model shrunk (body radius and the
+    150 m smoothing of the start model scale with nz / 800)."""
+    dx = 5.0
+    sc = nz / 800.0
     vp = _layered_with_fault(nx, nz, dx, 7, nlayer=10, vmax=4000.0, x_centre=nx * 0.35)
     vp2 = _layered_with_fault(nx, nz, dx, 7, nlayer=10, vmax=4000.0, offset_m=80.0, dip_deg=-55.0,
                               x_centre=nx * 0.7)
@@ -205,13 +208,13 @@ def c5(nshots=128, nrec=600, nt=6000):
     zz, xx = np.mgrid[0:nz, 0:nx].astype(float)
     ang = np.arctan2(zz - 0.6 * nz, xx - 0.5 * nx)
     r = np.hypot((zz - 0.6 * nz) / 0.6, xx - 0.5 * nx)
-    body = r < 120.0 * (1.0 + 0.25 * np.sin(3 * ang) + 0.1 * np.cos(5 * ang))
+    body = r < 120.0 * sc * (1.0 + 0.25 * np.sin(3 * ang) + 0.1 * np.cos(5 * ang))
     vp = np.where(body, 4500.0, vp)
     vs = vp / 1.73
     xs = np.linspace(20, nx - 21, nshots).round().astype(int)
     shots = [dict(nt=nt, src=dict(x=[int(x)], z=[2], comp=[2]), rec=_line(nx, nrec), amp=1.0) for x in xs]
     w = Workload("c5", nx, nz, dx, 5e-4, nt, 40, 4, 0, 8.0, vp, vs, _gardner(vp), shots)
-    sm = gaussian_filter(vp, 30.0, mode="nearest")
+    sm = gaussian_filter(vp, 30.0 * sc, mode="nearest")
     w.background = (sm, sm / 1.73, _gardner(sm))
     return w
 
