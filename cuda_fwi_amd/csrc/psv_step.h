@@ -86,6 +86,18 @@ template <typename T>
 hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s,
                        hipEvent_t e_mid = nullptr);
 
+// One kernel of a step: which 0 the stress kernel, 1 the velocity kernel
+// (with its point sources and receivers).  The reverse step of the
+// random-boundary reconstruction runs them in the opposite order with a
+// negated dt and negated source signals.
+template <typename T>
+hipError_t launch_half(const StepParams<T>& p, int order, int mode, int tz, int which, hipStream_t s);
+
+// The FWI window cells of the active shots' current state into store plane
+// 0 (v / dt and the stresses, as the forward stress kernel stores them).
+template <typename T>
+hipError_t launch_store_plane(const StepParams<T>& p, hipStream_t s);
+
 // The explosive (sxx + szz) sources of the active shots, added after a step.
 template <typename T>
 hipError_t launch_stress_sources(const StepParams<T>& p, hipStream_t s);

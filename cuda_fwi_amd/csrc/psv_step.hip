@@ -619,6 +619,72 @@ static hipError_t launch_one(const StepParams<T>& p, hipStream_t s, hipEvent_t e
     return hipGetLastError();
 }
 
+template <typename T, int ORDER, int MODE, int TZ>
+static hipError_t launch_half_one(const StepParams<T>& p, int which, hipStream_t s)
+{
+    const int blocks = p.tiles_x * p.tiles_z * p.nslots;
+    if (which == 0)
+        hipLaunchKernelGGL((psv_stress_kernel<T, ORDER, MODE, TZ>), dim3(blocks), dim3(kThreads), 0, s, p);
+    else
+        hipLaunchKernelGGL((psv_velocity_kernel<T, ORDER, TZ>), dim3(blocks), dim3(kThreads), 0, s, p);
+    return hipGetLastError();
+}
+
+// The window cells of the active shots' current state into store plane 0,
+// the values the forward stress kernel stores: v / dt and the stresses.
+template <typename T>
+__global__ void store_plane_kernel(const StepParams<T> p, int nzw, int nxw)
+{
+    const int slot = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nzw * nxw) return;
+    const int zf = i / nxw, xf = i - zf * nxw;
+    const int iz = p.fwi_z1 + zf * p.fwi_dz, ix = p.fwi_x1 + xf * p.fwi_dx;
+    const size_t c = (size_t)slot * p.fst + (size_t)iz * p.pitch + ix;
+    const size_t plane = (size_t)p.nfz * p.nfx;
+    T* st = p.store + (size_t)slot * p.sst + (size_t)zf * p.nfx + xf;
+    st[0 * plane] = p.vx[c] / p.dt;
+    st[1 * plane] = p.vz[c] / p.dt;
+    st[2 * plane] = p.sxx[c];
+    st[3 * plane] = p.szx[c];
+    st[4 * plane] = p.szz[c];
+}
+
+template <typename T>
+hipError_t launch_store_plane(const StepParams<T>& p, hipStream_t s)
+{
+    const int nzw = (p.fwi_z2 - p.fwi_z1 + p.fwi_dz - 1) / p.fwi_dz;
+    const int nxw = (p.fwi_x2 - p.fwi_x1 + p.fwi_dx - 1) / p.fwi_dx;
+    if (nzw <= 0 || nxw <= 0 || p.nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL((store_plane_kernel<T>), dim3((nzw * nxw + 255) / 256, p.nslots), dim3(256), 0, s, p,
+                       nzw, nxw);
+    return hipGetLastError();
+}
+
+template hipError_t launch_store_plane<float>(const StepParams<float>&, hipStream_t);
+template hipError_t launch_store_plane<double>(const StepParams<double>&, hipStream_t);
+
+#define PSV_HALF(O, M, Z) \
+    if (order == O && mode == M && tz == Z) return launch_half_one<T, O, M, Z>(p, which, s);
+
+template <typename T>
+hipError_t launch_half(const StepParams<T>& p, int order, int mode, int tz, int which, hipStream_t s)
+{
+    PSV_HALF(2, kForward, 4) PSV_HALF(2, kAdjoint, 4)
+    PSV_HALF(4, kForward, 4) PSV_HALF(4, kAdjoint, 4)
+    PSV_HALF(2, kForward, 8) PSV_HALF(2, kAdjoint, 8)
+    PSV_HALF(4, kForward, 8) PSV_HALF(4, kAdjoint, 8)
+    PSV_HALF(2, kForward, 16) PSV_HALF(2, kAdjoint, 16)
+    PSV_HALF(4, kForward, 16) PSV_HALF(4, kAdjoint, 16)
+    PSV_HALF(2, kForward, 32) PSV_HALF(2, kAdjoint, 32)
+    PSV_HALF(4, kForward, 32) PSV_HALF(4, kAdjoint, 32)
+    return hipErrorInvalidValue;
+}
+#undef PSV_HALF
+
+template hipError_t launch_half<float>(const StepParams<float>&, int, int, int, int, hipStream_t);
+template hipError_t launch_half<double>(const StepParams<double>&, int, int, int, int, hipStream_t);
+
 #define PSV_CASE(O, M, Z) \
     if (order == O && mode == M && tz == Z) return launch_one<T, O, M, Z>(p, s, e_mid);
 

@@ -14,6 +14,11 @@
 // (wavefields + CPML memory) every k steps; the adjoint then replays each
 // segment forward into a small segment store just before it needs it
 // (PSVFWI_STORE_CHECKPOINT).  Both give the same gradient bit for bit.
+// Random-boundary reconstruction (PSVFWI_STORE_RANDOM_BOUNDARY, an
+// extension): without CPML the scheme runs backward exactly up to rounding,
+// so the adjoint rebuilds the forward state step by step from the forward's
+// final state (reverse step: sources off, velocity kernel with -dt, stress
+// kernel with -dt) and keeps one store plane; nothing is stored in the forward.
 // Discrete gradient (PSVFWI_GRADIENT_DISCRETE, an extension): checkpoints of
 // the full state every k steps; the adjoint replays each segment keeping
 // every state of it and runs the transposed step (psv_discrete.hip) back
@@ -257,7 +262,7 @@ public:
 
     psvfwi_status set_store_mode(int mode, int interval) override
     {
-        if (mode != PSVFWI_STORE_FULL && mode != PSVFWI_STORE_CHECKPOINT)
+        if (mode != PSVFWI_STORE_FULL && mode != PSVFWI_STORE_CHECKPOINT && mode != PSVFWI_STORE_RANDOM_BOUNDARY)
             return fail(PSVFWI_E_ARG, "set_store_mode: unknown mode");
         if (mode == PSVFWI_STORE_CHECKPOINT && interval < 1)
             return fail(PSVFWI_E_ARG, "set_store_mode: checkpoint interval must be >= 1");
@@ -350,6 +355,8 @@ public:
         for (int c = 0; c < kNumCand; c++) {
             tune_ms_[c] = -1;
             if (!tile_ok(cand[c])) continue;
+            // the reverse steps of the random-boundary reconstruction need the split kernels
+            if (cand[c] < 0 && store_mode_ == PSVFWI_STORE_RANDOM_BOUNDARY) continue;
             tz_ = cand[c];
             PSV_TRY(ensure_all_tile_lists());
             PSV_TRY(zero_state(main_));
@@ -443,6 +450,8 @@ public:
         PSV_HIP(psv::launch_fill<T>(rec_.p, rec_.n, (T)0, stream_));
         const bool full = grad && !discrete && store_mode_ == PSVFWI_STORE_FULL;
         const bool ckpt = grad && (discrete || store_mode_ == PSVFWI_STORE_CHECKPOINT);
+        const bool rb = grad && !discrete && store_mode_ == PSVFWI_STORE_RANDOM_BOUNDARY;
+        if (rb) PSV_TRY(rb_check());
         const int kint = discrete ? disc_k_ : ck_k_;
         // the store is not cleared: the adjoint reads exactly the entries the
         // forward writes, and get_store zeroes the ones the window loops never
@@ -507,6 +516,7 @@ public:
         fwd_done_ = true;
         fwd_grad_done_ = grad;
         fwd_disc_ = discrete;
+        fwd_rb_ = rb;
         ck_used_k_ = kint;
         store_loaded_ = full;
         // with the fused kernel a shot that ended early holds its final state in
@@ -607,6 +617,7 @@ public:
         const bool energy = flags & PSVFWI_ENERGY;
         const bool grad = (flags & PSVFWI_GRADIENT) != 0;
         if (grad && grad_mode_ == PSVFWI_GRADIENT_DISCRETE) return adjoint_discrete();
+        if (grad && store_mode_ == PSVFWI_STORE_RANDOM_BOUNDARY) return adjoint_rb(flags);
         if (grad && !win_on_) return fail(PSVFWI_E_STATE, "adjoint: PSVFWI_GRADIENT needs a window");
         if (grad && fwd_disc_)
             return fail(PSVFWI_E_STATE, "adjoint: the forward ran for the discrete gradient; run it again");
@@ -703,6 +714,105 @@ public:
         ran_ = true;
         adj_grad_done_ = grad;
         parity_ = false;  // every shot ends at it = 0
+        return PSVFWI_OK;
+    }
+
+    // The conditions under which the step can be run backward: no CPML (its
+    // damping is not reversible; the random boundary layer of the model takes
+    // its place), the split kernels (they update in place), and no order-2
+    // free surface (the reference's override sets sxx from the velocities
+    // alone, so the previous sxx cannot be recovered).
+    psvfwi_status rb_check() const
+    {
+        if (g_.npml != 0)
+            return fail(PSVFWI_E_ARG, "random-boundary reconstruction needs npml = 0 (a random boundary "
+                                      "layer in the model instead of CPML)");
+        if (fused()) return fail(PSVFWI_E_ARG, "random-boundary reconstruction needs the split step (tile > 0)");
+        if (g_.fsurf && g_.fdorder == 2)
+            return fail(PSVFWI_E_ARG, "random-boundary reconstruction: the order-2 free surface is not reversible");
+        return PSVFWI_OK;
+    }
+
+    // Adjoint with the forward state rebuilt backward from the forward's
+    // final state: per step it = ntmax-1 .. 0, the reverse step u(it+1) ->
+    // u(it), the store plane of u(it) when it is a correlation step, then the
+    // adjoint step.
+    psvfwi_status adjoint_rb(int flags)
+    {
+        if (!fwd_grad_done_ || !fwd_rb_)
+            return fail(PSVFWI_E_STATE, "adjoint: run forward with PSVFWI_GRADIENT in the random-boundary store "
+                                        "mode first");
+        if (!win_on_) return fail(PSVFWI_E_STATE, "adjoint: PSVFWI_GRADIENT needs a window");
+        PSV_TRY(rb_check());
+        for (int i = 0; i < nslots_; i++)
+            if (!adj_ready_[(size_t)i] && !shots_[(size_t)i].rx.empty())
+                return fail(PSVFWI_E_STATE, "adjoint: no adjoint sources (call misfit or set_adjoint_sources)");
+        const bool energy = flags & PSVFWI_ENERGY;
+        // the reconstruction starts from the forward's final state
+        PSV_TRY(alloc_state(replay_));
+        {
+            psv::CopyList l{};
+            for (int f = 0; f < 5; f++) {
+                l.src[l.n] = main_.f[f].p;
+                l.dst[l.n] = replay_.f[f].p;
+                l.bytes[l.n] = (size_t)nslots_ * fst_ * sizeof(T);
+                l.n++;
+            }
+            PSV_HIP(psv::launch_copies(l, stream_));
+        }
+        // negated source signals for the reverse step
+        PSV_HIP(neg_sig_.alloc(std::max<size_t>(fwd_sig_h_.size(), 1)));
+        if (!fwd_sig_h_.empty()) {
+            std::vector<T> neg(fwd_sig_h_.size());
+            for (size_t i = 0; i < neg.size(); i++) neg[i] = -fwd_sig_h_[i];
+            PSV_HIP(hipMemcpy(neg_sig_.p, neg.data(), neg.size() * sizeof(T), hipMemcpyHostToDevice));
+        }
+        PSV_TRY(zero_state(main_));
+        PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(psv::launch_fill<T>(We_.p, We_.n, (T)0, stream_));
+        PSV_HIP(ux_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(uz_.alloc((size_t)nslots_ * fst_));
+        PSV_HIP(psv::launch_fill<T>(ux_.p, ux_.n, (T)0, stream_));
+        PSV_HIP(psv::launch_fill<T>(uz_.p, uz_.n, (T)0, stream_));
+        gst_ = (size_t)3 * nfz_ * nfx_;
+        PSV_HIP(grad_.alloc((size_t)nslots_ * gst_));
+        PSV_HIP(psv::launch_fill<T>(grad_.p, grad_.n, (T)0, stream_));
+        PSV_TRY(alloc_store(1));
+        PSV_TRY(ensure_all_tile_lists());
+        const int aflags = (energy ? psv::kEnergy : 0) | psv::kStore;
+
+        PSV_TRY(begin_samples());
+        PSV_HIP(hipEventRecord(ev0_, stream_));
+        launches_ = 0;
+        for (int it = ntmax_ - 1; it >= 0; it--) {
+            const int na = active(it);
+            psv::StepParams<T> r = base_params(replay_, fwd_tab_noreceivers_, true);
+            r.it = it;
+            r.nslots = na;
+            r.flags = 0;
+            r.We = nullptr;
+            r.dt = -r.dt;
+            r.sig = neg_sig_.p;
+            PSV_HIP(psv::launch_stress_sources<T>(r, stream_));
+            PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 1, stream_));
+            PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 0, stream_));
+            if (it % w_.dt == 0) {
+                psv::StepParams<T> q = base_params(replay_, fwd_tab_noreceivers_, true);
+                q.nslots = na;
+                PSV_HIP(psv::launch_store_plane<T>(q, stream_));
+            }
+            PSV_TRY(adjoint_step(it, aflags, it / w_.dt));
+        }
+        PSV_HIP(hipEventRecord(ev1_, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        float ms = 0;
+        PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        last_ms_ = ms;
+        PSV_TRY(collect_samples());
+        ran_ = true;
+        adj_grad_done_ = true;
+        fwd_grad_done_ = false;  // the forward's final state has been consumed
+        parity_ = false;
         return PSVFWI_OK;
     }
 
@@ -1147,6 +1257,7 @@ private:
                 for (double v : sh.ssig) sig.push_back((T)v);
             }
         }
+        if (with_signals) fwd_sig_h_ = sig;
         // inj_row holds signal rows relative to the shot; the kernel adds sig_off[slot]
         t.grp_off = off;
         t.grp_zx.clear();
@@ -1617,6 +1728,11 @@ private:
     // (lam, mu, mu_zx, rho_zp, rho_xp), segment states, host model copies
     int grad_mode_ = PSVFWI_GRADIENT_REFERENCE, disc_k_ = 64, ck_used_k_ = 200;
     bool fwd_disc_ = false, disc_done_ = false;
+    // random-boundary reconstruction: the forward ran for it; the host copy of
+    // the source signals and their negation on the device
+    bool fwd_rb_ = false;
+    std::vector<T> fwd_sig_h_;
+    DBuf<T> neg_sig_;
     State<T> adj_;
     DBuf<T> dw_[4], dg_[5], seg_;
     DBuf<int> dgrp_off_;

@@ -36,7 +36,7 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_average_parameters", "psvfwi_ricker", "psvfwi_set_snapshots",
            "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps",
            "psvfwi_set_gradient_mode", "psvfwi_get_coefficient_gradient", "psvfwi_get_model_gradient",
-           "psvfwi_chain_gradient")
+           "psvfwi_chain_gradient", "psvfwi_random_boundary")
 
 
 class PsvfwiError(RuntimeError):
@@ -140,7 +140,7 @@ class Plan:
         _check(lib().psvfwi_set_window(self._h, C.byref(w)))
 
     def set_store_mode(self, mode, interval=0):
-        m = {"full": STORE_FULL, "checkpoint": STORE_CHECKPOINT}.get(mode, mode)
+        m = {"full": STORE_FULL, "checkpoint": STORE_CHECKPOINT, "random_boundary": 2}.get(mode, mode)
         _check(lib().psvfwi_set_store_mode(self._h, m, int(interval)))
 
     def set_tile(self, tz):
@@ -344,6 +344,15 @@ def chain_gradient(mu, mu_zx, rho_zp, rho_xp, g):
     _check(lib().psvfwi_chain_gradient(int(nzt), int(nxt), *[_dptr(x) for x in arrs],
                                        *[_dptr(o) for o in out]))
     return out
+
+
+def random_boundary(lam, mu, rho, width, fsurf, seed=1, max_increase=0.5):
+    """Copies of lam, mu, rho with a random boundary layer (psvfwi_random_boundary)."""
+    lam, mu, rho = (np.array(a, np.float64, copy=True, order="C") for a in (lam, mu, rho))
+    nzt, nxt = lam.shape
+    _check(lib().psvfwi_random_boundary(int(nzt), int(nxt), int(width), int(fsurf), C.c_ulonglong(seed),
+                                        C.c_double(max_increase), _dptr(lam), _dptr(mu), _dptr(rho)))
+    return lam, mu, rho
 
 
 def ricker(nt, dt, amp, fc, ts=0.0):

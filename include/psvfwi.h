@@ -48,6 +48,13 @@ typedef enum psvfwi_precision { PSVFWI_FP32 = 0, PSVFWI_FP64 = 1 } psvfwi_precis
 /* how the adjoint gets the forward wavefield at the correlation steps */
 #define PSVFWI_STORE_FULL 0   /* decimated store of every fwi_dt-th step (reference) */
 #define PSVFWI_STORE_CHECKPOINT 1 /* full state every k steps, segments recomputed */
+/* No store: the adjoint rebuilds the forward state by running the scheme backward from the
+ * forward's final state (no counterpart in the reference, which stores).  Needs npml = 0
+ * with a random boundary layer in the model (psvfwi_random_boundary) in place of CPML, a
+ * positive tile height, and no order-2 free surface.  Device memory: one state and one
+ * store plane per shot.  The gradient equals the stored one up to the rounding of the
+ * reverse steps. */
+#define PSVFWI_STORE_RANDOM_BOUNDARY 2
 
 typedef struct psvfwi_grid_desc {
     int nzt, nxt;       /* padded grid (FDGrid::total_grid) */
@@ -117,7 +124,8 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
 
 /* Autotuner: times `steps` forward steps of the current batch with every
  * step kernel and tile height of psvfwi_set_tile, keeps the fastest and
- * returns its psvfwi_set_tile value in *best_tile (may be NULL).
+ * returns its psvfwi_set_tile value in *best_tile (may be NULL).  In the
+ * random-boundary store mode only the split kernels (tz > 0) are tried.
  * Needs set_model / set_pml / set_shots first.  (No counterpart in the
  * reference, whose launch shape is fixed at 16 x 16,
  * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */
@@ -239,6 +247,16 @@ psvfwi_status psvfwi_cpml_profiles(int npml, double dh, double fpml, double npow
  * FDMedium::average_parameters (CUDA_week2/fd_medium.cu:128-159) */
 psvfwi_status psvfwi_average_parameters(int nzt, int nxt, const double* mu, const double* rho,
                                         double* mu_zx, double* rho_zp, double* rho_xp);
+
+/* Random boundary layer for PSVFWI_STORE_RANDOM_BOUNDARY (no counterpart in the reference):
+ * in place on lam, mu, rho [nzt][nxt], the cells within `width` of the left, right and bottom
+ * edges (and the top one when fsurf == 0) get vp and vs scaled by 1 + max_increase * (d /
+ * width) * u, d the depth into the layer and u uniform in [0, 1) from a hash of (seed, iz,
+ * ix); lam and mu scale by the square, rho is kept.  The waves that reach the layer scatter
+ * incoherently instead of being absorbed.  The time step must stay stable for the largest
+ * velocity, (1 + max_increase) times the model's. */
+psvfwi_status psvfwi_random_boundary(int nzt, int nxt, int width, int fsurf, unsigned long long seed,
+                                     double max_increase, double* lam, double* mu, double* rho);
 
 /* Ricker wavelet: create_wavelet shape 1 (CUDA_week2/fd_seismogram.cu:166-186) */
 psvfwi_status psvfwi_ricker(int nt, double dt, double amp, double fc, double ts, double* signal);
