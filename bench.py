@@ -449,6 +449,7 @@ def run_c4(args):
     plan.set_graph_steps(args.graph_steps)
     plan.set_model(lt, mt, *P.average_parameters(mt, rt))
     tile = args.tile if args.tile != 0 else plan.autotune(min(200, w.nt))
+    band = plan.shot_order()
     for g in groups:
         plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
                         for i in g])
@@ -482,6 +483,7 @@ def run_c4(args):
         plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
                         for i in g])
         plan.set_tile(tile)
+        plan.set_shot_order(band)
         ts = time.perf_counter() - tg
         fwd_adj(plan, g)
         _sync()
@@ -529,7 +531,8 @@ def run_c4(args):
                                       f"groups sized to {args.store_budget_gb} GB of store"
                                       if not recompute else f"checkpoint every {args.interval} steps, "
                                       "segments recomputed (recomputation not counted)"),
-                   "tile": tile, "graph_steps": args.graph_steps},
+                   "tile": tile, "shot_order": "shot-major" if band == 0 else f"bands of {band} tile rows",
+                   "graph_steps": args.graph_steps},
         "hbm": {"alg_bytes": int(alg), "alg_GBps": round(alg / tb / 1e9, 1),
                 "frac_of_peak": round(alg / tb / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "algorithmic bytes of forward + adjoint + gradient over the wall time"},

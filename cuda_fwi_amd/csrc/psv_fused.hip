@@ -84,8 +84,18 @@ __global__ __launch_bounds__(kThreads) void psv_fused_kernel(const StepParams<T>
 
     const int ntile = p.tiles_x * p.tiles_z;
     const int bid = xcd_band(blockIdx.x, ntile * p.nslots);
-    const int slot = bid / ntile;
-    const int tile = bid - slot * ntile;
+    int slot, tile;
+    if (p.band > 0) {
+        const int per_band = p.band * p.tiles_x * p.nslots;
+        const int b = bid / per_band;
+        const int rem = bid - b * per_band;
+        const int tb = min(p.band, p.tiles_z - b * p.band) * p.tiles_x;
+        slot = rem / tb;
+        tile = b * p.band * p.tiles_x + (rem - slot * tb);
+    } else {
+        slot = bid / ntile;
+        tile = bid - slot * ntile;
+    }
     const int tzi = tile / p.tiles_x;
     const int z0 = tzi * TZ, x0 = (tile - tzi * p.tiles_x) * kTX;
     const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;

@@ -22,6 +22,8 @@ struct StepParams {
     int nzt, nxt, pitch;
     int ppad, npml, tp, fsurf, pnx, pnz;
     int tiles_x, tiles_z, nslots;
+    int band;            // block order: 0 shot-major (slot, tile); k > 0 bands of k tile rows,
+                         // within a band the shots one after another (band, slot, tile)
     int it;              // time index (plus *it_dev when it_dev != nullptr)
     const int* it_dev;   // device-side time counter for graph replays
     int flags;           // kStore | kEnergy

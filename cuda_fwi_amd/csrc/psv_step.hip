@@ -55,8 +55,19 @@ __device__ __forceinline__ TileId tile_of(const StepParams<T>& p)
     const int ntile = p.tiles_x * p.tiles_z;
     const int bid = xcd_remap(blockIdx.x, ntile * p.nslots);
     TileId t;
-    t.slot = bid / ntile;
-    t.tile = bid - t.slot * ntile;
+    if (p.band > 0) {
+        // bands of p.band tile rows; the shots of a band run one after
+        // another, so the band's model tiles can be L2 hits after the first
+        const int per_band = p.band * p.tiles_x * p.nslots;
+        const int b = bid / per_band;
+        const int rem = bid - b * per_band;
+        const int tb = min(p.band, p.tiles_z - b * p.band) * p.tiles_x;
+        t.slot = rem / tb;
+        t.tile = b * p.band * p.tiles_x + (rem - t.slot * tb);
+    } else {
+        t.slot = bid / ntile;
+        t.tile = bid - t.slot * ntile;
+    }
     const int tzi = t.tile / p.tiles_x;
     t.z0 = tzi * TZ;
     t.x0 = (t.tile - tzi * p.tiles_x) * kTX;

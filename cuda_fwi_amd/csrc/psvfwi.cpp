@@ -122,6 +122,8 @@ struct psvfwi_plan {
     virtual psvfwi_status set_shots(int, const psvfwi_shot_desc*) = 0;
     virtual psvfwi_status set_tile(int) = 0;
     virtual psvfwi_status set_graph_steps(int) = 0;
+    virtual psvfwi_status set_shot_order(int) = 0;
+    virtual psvfwi_status shot_order(int*) const = 0;
     virtual psvfwi_status set_snapshots(int, int) = 0;
     virtual psvfwi_status get_snapshot(int, int, double*) const = 0;
     virtual psvfwi_status snapshot_count(int, int*) const = 0;
@@ -349,6 +351,47 @@ public:
         PSV_TRY(alloc_state(main_));
         PSV_TRY(alloc_state(main2_));
         PSV_HIP(We_.alloc((size_t)nslots_ * fst_));
+        // with a window, the candidates are timed on forward steps and on
+        // adjoint steps with the gradient correlation (one store plane), the
+        // two kinds of step of a gradient; without one on forward steps
+        const bool adj = win_on_;
+        if (adj) {
+            PSV_HIP(ux_.alloc((size_t)nslots_ * fst_));
+            PSV_HIP(uz_.alloc((size_t)nslots_ * fst_));
+            gst_ = (size_t)3 * nfz_ * nfx_;
+            PSV_HIP(grad_.alloc((size_t)nslots_ * gst_));
+            PSV_TRY(alloc_store(1));
+        }
+        const int nstep = std::min(steps, ntmax_);
+        auto time_steps = [&](float* out) -> psvfwi_status {
+            PSV_TRY(ensure_all_tile_lists());
+            float ms = 0;
+            for (int rep = 0; rep < 2; rep++) {  // the first repetition warms up
+                PSV_TRY(zero_state(main_));
+                PSV_TRY(zero_state(main2_));
+                PSV_HIP(hipEventRecord(ev0_, stream_));
+                for (int it = 0; it < nstep; it++) {
+                    psv::StepParams<T> p = base_params(main_, fwd_tab_noreceivers_, true);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = 0;
+                    PSV_TRY(launch_plain(p, psv::kForward, main_, main2_));
+                }
+                for (int it = 0; adj && it < nstep; it++) {
+                    psv::StepParams<T> p = base_params(main_, adj_tab_, false);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = psv::kStore;
+                    p.tf0 = it / w_.dt;
+                    PSV_TRY(launch_plain(p, psv::kAdjoint, main_, main2_));
+                }
+                PSV_HIP(hipEventRecord(ev1_, stream_));
+                PSV_HIP(hipStreamSynchronize(stream_));
+                PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            }
+            *out = ms;
+            return PSVFWI_OK;
+        };
         const int cand[kNumCand] = {4, 8, 16, 32, -8, -16};
         float best_ms = 1e30f;
         int best_tz = tz_;
@@ -358,23 +401,8 @@ public:
             // the reverse steps of the random-boundary reconstruction need the split kernels
             if (cand[c] < 0 && store_mode_ == PSVFWI_STORE_RANDOM_BOUNDARY) continue;
             tz_ = cand[c];
-            PSV_TRY(ensure_all_tile_lists());
-            PSV_TRY(zero_state(main_));
-            PSV_TRY(zero_state(main2_));
-            for (int rep = 0; rep < 2; rep++) {  // first repetition warms up
-                PSV_HIP(hipEventRecord(ev0_, stream_));
-                for (int it = 0; it < std::min(steps, ntmax_); it++) {
-                    psv::StepParams<T> p = base_params(main_, fwd_tab_noreceivers_, true);
-                    p.it = it;
-                    p.nslots = active(it);
-                    p.flags = 0;
-                    PSV_TRY(launch_plain(p, psv::kForward, main_, main2_));
-                }
-                PSV_HIP(hipEventRecord(ev1_, stream_));
-                PSV_HIP(hipStreamSynchronize(stream_));
-            }
             float ms = 0;
-            PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+            PSV_TRY(time_steps(&ms));
             tune_ms_[c] = ms;
             if (ms < best_ms) {
                 best_ms = ms;
@@ -382,8 +410,39 @@ public:
             }
         }
         tz_ = best_tz;
+        // the shot-to-workgroup order of a batch: shot-major, or bands of 1 or
+        // 4 tile rows with the shots of a band together
+        if (nslots_ > 1) {
+            int best_band = band_;
+            const int bands[3] = {0, 1, 4};
+            for (int b : bands) {
+                band_ = b;
+                float ms = 0;
+                PSV_TRY(time_steps(&ms));
+                if (b == 0 || ms < best_ms) {
+                    best_ms = ms;
+                    best_band = b;
+                }
+            }
+            band_ = best_band;
+        }
         if (best) *best = best_tz;
         ran_ = false;
+        fwd_grad_done_ = false;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_shot_order(int band) override
+    {
+        if (band < 0) return fail(PSVFWI_E_ARG, "set_shot_order: band must be >= 0");
+        band_ = band;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status shot_order(int* band) const override
+    {
+        if (!band) return fail(PSVFWI_E_ARG, "shot_order: null argument");
+        *band = band_;
         return PSVFWI_OK;
     }
 
@@ -1601,6 +1660,7 @@ private:
         p.pnz = pnz_;
         p.tiles_x = (g_.nxt + psv::kTX - 1) / psv::kTX;
         p.tiles_z = (g_.nzt + tz_abs() - 1) / tz_abs();
+        p.band = band_;
         p.org = org_;
         p.it_dev = nullptr;
         p.dt = (T)g_.dt;
@@ -1686,6 +1746,7 @@ private:
     int fpad_ = 1, ppad_ = 0, tp_ = 2, pitch_ = 0, rows_ = 0, org_ = 0, pnx_ = 0, pnz_ = 0, tz_ = 16;
     bool parity_ = false;
     int graph_steps_ = 0;  // > 0: time loops replay captured chunks of this many steps
+    int band_ = 0;         // block order of a batch (StepParams::band)
     DBuf<int> it_dev_;
     size_t fst_ = 0, xst_ = 0, zst_ = 0;
     hipStream_t stream_ = nullptr;
@@ -1857,6 +1918,18 @@ psvfwi_status psvfwi_set_graph_steps(psvfwi_plan* plan, int steps)
 {
     if (!plan) return fail(PSVFWI_E_ARG, "null plan");
     return plan->set_graph_steps(steps);
+}
+
+psvfwi_status psvfwi_set_shot_order(psvfwi_plan* plan, int band)
+{
+    PSV_PLAN(plan);
+    return plan->set_shot_order(band);
+}
+
+psvfwi_status psvfwi_get_shot_order(const psvfwi_plan* plan, int* band)
+{
+    PSV_PLAN(plan);
+    return plan->shot_order(band);
 }
 
 psvfwi_status psvfwi_set_snapshots(psvfwi_plan* plan, int interval, int field)

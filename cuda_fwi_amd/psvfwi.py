@@ -36,7 +36,8 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_average_parameters", "psvfwi_ricker", "psvfwi_set_snapshots",
            "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps",
            "psvfwi_set_gradient_mode", "psvfwi_get_coefficient_gradient", "psvfwi_get_model_gradient",
-           "psvfwi_chain_gradient", "psvfwi_random_boundary")
+           "psvfwi_chain_gradient", "psvfwi_random_boundary", "psvfwi_set_shot_order",
+           "psvfwi_get_shot_order")
 
 
 class PsvfwiError(RuntimeError):
@@ -170,6 +171,15 @@ class Plan:
         _check(lib().psvfwi_set_shots(self._h, len(descs), arr))
         self._shots = [dict(nt=int(s["nt"]), nrec=len((s.get("rec") or {"x": []})["x"]))
                        for s in shots]
+
+    def set_shot_order(self, band):
+        """0: shot-major workgroups; k > 0: bands of k tile rows, shots together within a band."""
+        _check(lib().psvfwi_set_shot_order(self._h, int(band)))
+
+    def shot_order(self):
+        b = C.c_int()
+        _check(lib().psvfwi_get_shot_order(self._h, C.byref(b)))
+        return b.value
 
     def set_graph_steps(self, steps):
         """Replay time steps as captured HIP graphs of `steps` launches (0: off)."""

@@ -124,12 +124,20 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
 
 /* Autotuner: times `steps` forward steps of the current batch with every
  * step kernel and tile height of psvfwi_set_tile, keeps the fastest and
- * returns its psvfwi_set_tile value in *best_tile (may be NULL).  In the
+ * returns its psvfwi_set_tile value in *best_tile (may be NULL); for a batch it
+ * then chooses the shot order (psvfwi_set_shot_order) the same way.  In the
  * random-boundary store mode only the split kernels (tz > 0) are tried.
  * Needs set_model / set_pml / set_shots first.  (No counterpart in the
  * reference, whose launch shape is fixed at 16 x 16,
  * CUDA_week2/forward_kernel_PSV_GPU.cu:543-545.) */
 psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile);
+
+/* Order of the workgroups of a batched launch (no counterpart in the reference, which runs
+ * one shot per launch): band 0 = shot-major (every tile of shot 0, then shot 1, ...); band
+ * k > 0 = bands of k tile rows, and within a band the shots one after another.  psvfwi_autotune
+ * chooses among 0, 1 and 4 for batches of more than one shot.  Results do not depend on it. */
+psvfwi_status psvfwi_set_shot_order(psvfwi_plan* plan, int band);
+psvfwi_status psvfwi_get_shot_order(const psvfwi_plan* plan, int* band);
 
 /* Graph-captured time loop: with steps > 0 (even), forward and adjoint
  * runs replay HIP graphs of `steps` consecutive step launches (the step index
