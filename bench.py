@@ -350,17 +350,24 @@ def run_c5(args):
     plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
                   dz=w.dx, hc=hc, precision=args.precision)
     plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt))
-    plan.set_shots([w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in w.shots])
+    shots = [w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in w.shots]
     if args.tile:
         plan.set_tile(args.tile)
     truth = w.padded()
     plan.set_model(truth[0], truth[1], *P.average_parameters(truth[1], truth[2]))
-    plan.forward(energy=False)
-    observed = [plan.records(i) for i in range(nshots)]
-    _log(f"c5 at scale {sc}: {nx} x {nz}, {nshots} shots x {nrec} receivers, nt {nt}; observed data done")
+    g = args.c5_group or nshots
+    observed = []
+    for i in range(0, nshots, g):
+        plan.set_shots(shots[i:i + g])
+        plan.forward(energy=False)
+        observed += [plan.records(k) for k in range(len(shots[i:i + g]))]
+        _log(f"observed traces of shots {i}-{min(i + g, nshots) - 1}")
+    _log(f"c5 at scale {sc}: {nx} x {nz}, {nshots} shots x {nrec} receivers, nt {nt}, groups of {g}; "
+         "observed data done")
     start = w.padded(*w.background)
     win = (w.zoff, w.zoff + nz, w.ppad, w.ppad + nx)
-    inv = Inversion(plan, observed, start, win, segment=args.interval, mute=args.c5_mute)
+    inv = Inversion(plan, shots, observed, start, win, segment=args.interval, mute=args.c5_mute,
+                    group=args.c5_group or None)
     err0 = inv.model_error(truth)
     _sync()
     t0 = time.perf_counter()
@@ -597,6 +604,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c3mem", "c5"])
     ap.add_argument("--iters", type=int, default=20, help="c5: FWI iterations")
+    ap.add_argument("--c5-group", type=int, default=0,
+                    help="c5: shots per batched launch (0: all); the discrete adjoint of all 128 full-size "
+                         "shots does not fit in HBM at once")
     ap.add_argument("--c5-mute", type=int, default=10,
                     help="c5: model rows below the surface with no update (sources and receivers at row 2)")
     ap.add_argument("--c5-scale", type=float, default=0.5,

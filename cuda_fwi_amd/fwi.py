@@ -20,7 +20,10 @@ with the batched, window-correct choices of the C++ driver
   padding is refilled by edge replication of the model region.
 
 Everything that touches the wavefields runs on the GPU through the plan; this
-module only moves the model (three grids per iteration) and scalars.
+module only moves the model (three grids per iteration) and scalars.  Shots
+can run in groups (one batched launch per group, the gradients and
+illuminations summed over groups) when the discrete adjoint's checkpoints and
+segment states of all shots do not fit in HBM together.
 """
 from __future__ import annotations
 
@@ -75,30 +78,67 @@ def _parabola(L2, sl):
 
 
 class Inversion:
-    """plan: a device plan with pml and shots set (observed traces are set
-    here); model: lam, mu, rho on the padded grid; win: (z1, z2, x1, x2), the
-    cells the inversion updates."""
+    """plan: a device plan with pml set; shots: the shot dicts of
+    psvfwi.Plan.set_shots; observed: one [nrec, nt] array per shot; model:
+    lam, mu, rho on the padded grid; win: (z1, z2, x1, x2), the cells the
+    inversion updates; group: shots per batched launch (None: all)."""
 
-    def __init__(self, plan: P.Plan, observed, model, win, segment=100, step=0.01, mute=0):
+    def __init__(self, plan: P.Plan, shots, observed, model, win, segment=100, step=0.01, mute=0, group=None):
         self.plan = plan
         self.win = win
         self.step = step
         self.mute = int(mute)
         self.m = {"lam": model[0].copy(), "mu": model[1].copy(), "rho": model[2].copy()}
-        self.nshots = len(observed)
-        plan.set_observed(observed)
+        self.shots = list(shots)
+        self.observed = list(observed)
+        n = len(self.shots)
+        g = n if not group else int(group)
+        self.groups = [list(range(i, min(i + g, n))) for i in range(0, n, g)]
+        self.current = None
+        self.log = None   # progress callback: one line per group of a gradient or trial
         plan.set_gradient_mode("discrete", segment)
         self.forwards = 0
         self.adjoints = 0
 
+    def _use(self, gi):
+        if self.current != gi:
+            ids = self.groups[gi]
+            self.plan.set_shots([self.shots[i] for i in ids])
+            self.plan.set_observed([self.observed[i] for i in ids])
+            self.current = gi
+
     def _set(self, m):
         self.plan.set_model(m["lam"], m["mu"], *P.average_parameters(m["mu"], m["rho"]))
 
-    def _misfit(self, m, gradient=False):
+    def _misfit(self, m):
         self._set(m)
-        self.plan.forward(energy=gradient, gradient=gradient)
-        self.forwards += 1
-        return float(np.sum(self.plan.misfit()))
+        total = 0.0
+        for gi in range(len(self.groups)):
+            self._use(gi)
+            self.plan.forward(energy=False, gradient=False)
+            self.forwards += 1
+            total += float(np.sum(self.plan.misfit()))
+            if self.log and len(self.groups) > 1:
+                self.log(f"  trial forward, group {gi + 1}/{len(self.groups)}")
+        return total
+
+    def _gradient(self):
+        """Misfit, summed illumination and dJ/d(lam, mu, rho) of the current model."""
+        self._set(self.m)
+        L2, ill, g = 0.0, 0.0, None
+        for gi, ids in enumerate(self.groups):
+            self._use(gi)
+            self.plan.forward(energy=True, gradient=True)
+            self.forwards += 1
+            L2 += float(np.sum(self.plan.misfit()))
+            ill = ill + sum(self.plan.field("We", i) for i in range(len(ids)))
+            self.plan.adjoint(energy=False, gradient=True)
+            self.adjoints += 1
+            gg = self.plan.model_gradient(-1)
+            g = gg if g is None else [a + b for a, b in zip(g, gg)]
+            if self.log and len(self.groups) > 1:
+                self.log(f"  gradient, group {gi + 1}/{len(self.groups)}")
+        return L2, ill, g
 
     def _trial(self, d, est):
         m = self.m
@@ -107,11 +147,7 @@ class Inversion:
 
     def iterate(self):
         """One iteration; returns (misfit before the update, step taken, trials)."""
-        L2 = self._misfit(self.m, gradient=True)
-        ill = sum(self.plan.field("We", i) for i in range(self.nshots))
-        self.plan.adjoint(energy=False, gradient=True)
-        self.adjoints += 1
-        g = self.plan.model_gradient(-1)
+        L2, ill, g = self._gradient()
         z1, z2, x1, x2 = self.win
         w = ill + 0.005 * ill[z1:z2, x1:x2].max()
         d = {k: -gk / w for k, gk in zip(("lam", "mu", "rho"), g)}
@@ -149,6 +185,7 @@ class Inversion:
         return L2, opt, ntrial
 
     def run(self, iters, truth=None, log=None):
+        self.log = log
         h = History()
         for i in range(iters):
             t0 = time.perf_counter()

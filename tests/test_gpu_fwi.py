@@ -19,21 +19,37 @@ def _setup():
     plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
                   dz=w.dx, hc=P.holberg(4, 1), precision="fp64")
     plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt))
-    plan.set_shots([w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in w.shots])
+    shots = [w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in w.shots]
+    plan.set_shots(shots)
     truth = w.padded()
     plan.set_model(truth[0], truth[1], *P.average_parameters(truth[1], truth[2]))
     plan.forward(energy=False)
     observed = [plan.records(i) for i in range(len(w.shots))]
     win = (w.zoff, w.zoff + w.nz, w.ppad, w.ppad + w.nx)
-    return w, plan, observed, truth, win
+    return w, plan, shots, observed, truth, win
 
 
 def test_fwi_discrete_gradient_reduces_misfit_and_model_error():
     print("host FWI loop, 240 x 80, 6 shots, 4 iterations", flush=True)
-    w, plan, observed, truth, win = _setup()
-    inv = Inversion(plan, observed, w.padded(*w.background), win, segment=50)
+    w, plan, shots, observed, truth, win = _setup()
+    inv = Inversion(plan, shots, observed, w.padded(*w.background), win, segment=50)
     e0 = inv.model_error(truth)
     h = inv.run(4, truth=truth, log=print)
     print(f"  misfit {h.misfit}; vp error {e0:.5f} -> {h.model_error}")
     assert all(b < a for a, b in zip(h.misfit, h.misfit[1:]))
     assert h.model_error[-1] < e0
+
+
+def test_fwi_shot_groups_equal_one_batch():
+    """Running the shots in groups of 4 and 1 gives the same iterations as
+    one batch (misfits and model to FP64 summation order)."""
+    print("host FWI loop in shot groups", flush=True)
+    runs = []
+    for group in (None, 4, 1):
+        w, plan, shots, observed, truth, win = _setup()
+        inv = Inversion(plan, shots, observed, w.padded(*w.background), win, segment=50, group=group)
+        h = inv.run(2)
+        runs.append((h.misfit, inv.vp()))
+    for mis, vp in runs[1:]:
+        assert np.allclose(mis, runs[0][0], rtol=1e-12, atol=0)
+        assert np.allclose(vp, runs[0][1], rtol=1e-9, atol=0)
