@@ -351,11 +351,18 @@ def run_c5(args):
                   dz=w.dx, hc=hc, precision=args.precision)
     plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt))
     shots = [w.padded_shot(s, P.ricker(s["nt"], w.dt, s["amp"], w.fc)[None]) for s in w.shots]
-    if args.tile:
-        plan.set_tile(args.tile)
     truth = w.padded()
     plan.set_model(truth[0], truth[1], *P.average_parameters(truth[1], truth[2]))
     g = args.c5_group or nshots
+    if args.tile:
+        plan.set_tile(args.tile)
+        plan.tile = args.tile
+    else:
+        plan.set_shots(shots[:g])
+        plan.tile = plan.autotune(min(100, nt))
+    if args.launch_group >= 0:
+        plan.set_launch_group(args.launch_group)
+    _log(f"tile {plan.tile}, shot order {plan.shot_order()}, launch group {plan.launch_group()}")
     observed = []
     for i in range(0, nshots, g):
         plan.set_shots(shots[i:i + g])
@@ -386,7 +393,8 @@ def run_c5(args):
         "config": {"workload": f"FWI {nx}x{nz} dx5 order-4 npml40, {nshots} shots x {nrec} receivers, nt {nt}, "
                                f"8 Hz, {args.iters} iterations, discrete gradient, checkpoints every "
                                f"{args.interval} steps, top {args.c5_mute} model rows not updated",
-                   "scale_of_2400x800_128_shots": sc},
+                   "scale_of_2400x800_128_shots": sc, "tile": plan.tile,
+                   "launch_group": plan.launch_group() or "whole batch"},
         "fwi": {"misfit": h.misfit, "vp_rel_error_start": err0, "vp_rel_error": h.model_error,
                 "step": h.step, "trial_forwards": h.trials, "seconds": [round(x, 2) for x in h.seconds],
                 "note": "value counts forward + adjoint cell updates of each iteration; trial forwards "
@@ -457,6 +465,9 @@ def run_c4(args):
     plan.set_model(lt, mt, *P.average_parameters(mt, rt))
     tile = args.tile if args.tile != 0 else plan.autotune(min(200, w.nt))
     band = plan.shot_order()
+    if args.launch_group >= 0:
+        plan.set_launch_group(args.launch_group)
+    lgroup = plan.launch_group()
     for g in groups:
         plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
                         for i in g])
@@ -465,7 +476,7 @@ def run_c4(args):
         for k, i in enumerate(g):
             obs[i] = plan.records(k)
     plan.set_model(lam, mu, *avg0)
-    _log(f"observed traces done, tile {tile}")
+    _log(f"observed traces done, tile {tile}, shot order {band}, launch group {lgroup}")
 
     def fwd_adj(pl, shot_ids):
         pl.set_observed([obs[i] for i in shot_ids])
@@ -539,7 +550,7 @@ def run_c4(args):
                                       if not recompute else f"checkpoint every {args.interval} steps, "
                                       "segments recomputed (recomputation not counted)"),
                    "tile": tile, "shot_order": "shot-major" if band == 0 else f"bands of {band} tile rows",
-                   "graph_steps": args.graph_steps},
+                   "launch_group": lgroup or "whole batch", "graph_steps": args.graph_steps},
         "hbm": {"alg_bytes": int(alg), "alg_GBps": round(alg / tb / 1e9, 1),
                 "frac_of_peak": round(alg / tb / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "algorithmic bytes of forward + adjoint + gradient over the wall time"},
@@ -627,6 +638,8 @@ def main():
                     help="replay the time loop as captured HIP graphs of this many steps (0: direct launches)")
     ap.add_argument("--gradient", default="reference", choices=["reference", "discrete"],
                     help="default workload: the reference's correlation gradient or the discrete (exact) one")
+    ap.add_argument("--launch-group", type=int, default=-1,
+                    help="c4: shots per launch group (-1: the autotuner's choice, 0: whole batch)")
     ap.add_argument("--store-budget-gb", type=float, default=200.0,
                     help="batched workload with --store full: store memory per group of shots")
     args = ap.parse_args()

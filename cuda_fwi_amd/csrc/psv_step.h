@@ -76,9 +76,11 @@ struct StepParams {
     const int* tinj_idx;     // injection group ids
     const int* trec_off;     // [nslots * ntile + 1] into trec_idx
     const int* trec_idx;     // receiver ids (global index into rec_zxc)
-    // explosive (stress) source groups: pairs (group id, slot)
+    // explosive (stress) source groups: pairs (group id, slot of the whole
+    // batch); slot0 is the batch slot of this launch's slot 0
     int nexpl;
     const int* expl_grp;
+    int slot0;
 };
 
 // Launches one step (stress kernel, velocity kernel, and the stress-source

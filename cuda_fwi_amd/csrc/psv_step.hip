@@ -586,8 +586,8 @@ __global__ void psv_stress_source_kernel(const StepParams<T> p)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.nexpl) return;
-    const int gid = p.expl_grp[2 * i], slot = p.expl_grp[2 * i + 1];
-    if (slot >= p.nslots) return;
+    const int gid = p.expl_grp[2 * i], slot = p.expl_grp[2 * i + 1] - p.slot0;
+    if (slot < 0 || slot >= p.nslots) return;
     const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;
     const int nt_s = p.shot_nt[slot];
     if (it >= nt_s) return;

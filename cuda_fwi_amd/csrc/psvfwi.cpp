@@ -124,6 +124,8 @@ struct psvfwi_plan {
     virtual psvfwi_status set_graph_steps(int) = 0;
     virtual psvfwi_status set_shot_order(int) = 0;
     virtual psvfwi_status shot_order(int*) const = 0;
+    virtual psvfwi_status set_launch_group(int) = 0;
+    virtual psvfwi_status launch_group(int*) const = 0;
     virtual psvfwi_status set_snapshots(int, int) = 0;
     virtual psvfwi_status get_snapshot(int, int, double*) const = 0;
     virtual psvfwi_status snapshot_count(int, int*) const = 0;
@@ -362,29 +364,34 @@ public:
             PSV_HIP(grad_.alloc((size_t)nslots_ * gst_));
             PSV_TRY(alloc_store(1));
         }
-        const int nstep = std::min(steps, ntmax_);
         auto time_steps = [&](float* out) -> psvfwi_status {
             PSV_TRY(ensure_all_tile_lists());
+            AllSlotsAfter restore{this};
             float ms = 0;
             for (int rep = 0; rep < 2; rep++) {  // the first repetition warms up
                 PSV_TRY(zero_state(main_));
                 PSV_TRY(zero_state(main2_));
                 PSV_HIP(hipEventRecord(ev0_, stream_));
-                for (int it = 0; it < nstep; it++) {
-                    psv::StepParams<T> p = base_params(main_, fwd_tab_noreceivers_, true);
-                    p.it = it;
-                    p.nslots = active(it);
-                    p.flags = 0;
-                    PSV_TRY(launch_plain(p, psv::kForward, main_, main2_));
+                for (int w0 = 0; w0 < nslots_; w0 += group_size()) {
+                    set_group(w0);
+                    const int nstep = std::min(steps, group_nt());
+                    for (int it = 0; it < nstep; it++) {
+                        psv::StepParams<T> p = base_params(main_, fwd_tab_noreceivers_, true);
+                        p.it = it;
+                        p.nslots = active(it);
+                        p.flags = 0;
+                        PSV_TRY(launch_plain(p, psv::kForward, main_, main2_));
+                    }
+                    for (int it = 0; adj && it < nstep; it++) {
+                        psv::StepParams<T> p = base_params(main_, adj_tab_, false);
+                        p.it = it;
+                        p.nslots = active(it);
+                        p.flags = psv::kStore;
+                        p.tf0 = it / w_.dt;
+                        PSV_TRY(launch_plain(p, psv::kAdjoint, main_, main2_));
+                    }
                 }
-                for (int it = 0; adj && it < nstep; it++) {
-                    psv::StepParams<T> p = base_params(main_, adj_tab_, false);
-                    p.it = it;
-                    p.nslots = active(it);
-                    p.flags = psv::kStore;
-                    p.tf0 = it / w_.dt;
-                    PSV_TRY(launch_plain(p, psv::kAdjoint, main_, main2_));
-                }
+                all_slots();
                 PSV_HIP(hipEventRecord(ev1_, stream_));
                 PSV_HIP(hipStreamSynchronize(stream_));
                 PSV_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
@@ -425,6 +432,26 @@ public:
                 }
             }
             band_ = best_band;
+            // shots per launch group (split step only): the whole batch, or
+            // groups small enough for their wavefields to stay in the
+            // Infinity Cache between steps
+            if (!fused()) {
+                int best_lg = 0;
+                const int sizes[6] = {16, 12, 8, 6, 4, 2};
+                lg_ = 0;
+                PSV_TRY(time_steps(&best_ms));
+                for (int k : sizes) {
+                    if (k >= nslots_) continue;
+                    lg_ = k;
+                    float ms = 0;
+                    PSV_TRY(time_steps(&ms));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best_lg = k;
+                    }
+                }
+                lg_ = best_lg;
+            }
         }
         if (best) *best = best_tz;
         ran_ = false;
@@ -436,6 +463,20 @@ public:
     {
         if (band < 0) return fail(PSVFWI_E_ARG, "set_shot_order: band must be >= 0");
         band_ = band;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status set_launch_group(int shots) override
+    {
+        if (shots < 0) return fail(PSVFWI_E_ARG, "set_launch_group: shots must be >= 0");
+        lg_ = shots;
+        return PSVFWI_OK;
+    }
+
+    psvfwi_status launch_group(int* shots) const override
+    {
+        if (!shots) return fail(PSVFWI_E_ARG, "launch_group: null argument");
+        *shots = lg_;
         return PSVFWI_OK;
     }
 
@@ -475,6 +516,7 @@ public:
         }
         shots_.swap(shots);
         nslots_ = n;
+        all_slots();
         shot_of_slot_.resize((size_t)n);
         std::iota(shot_of_slot_.begin(), shot_of_slot_.end(), 0);
         std::stable_sort(shot_of_slot_.begin(), shot_of_slot_.end(),
@@ -527,43 +569,49 @@ public:
         launches_ = 0;
         const int fflags = (energy ? psv::kEnergy : 0) | (full ? psv::kStore : 0);
         std::vector<hipGraphExec_t> graphs;
-        for (int it = 0; it < ntmax_; it++) {
-            const int na = active(it);
-            if (graph_steps_ > 0 && !snap && !ckpt) {
-                // a run of steps with the same active shots: replay a captured
-                // chunk of graph_steps_ steps, the step index read from it_dev_
-                int end = it;
-                while (end < ntmax_ && active(end) == na) end++;
-                if (end - it >= graph_steps_) {
-                    psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
-                    p.nslots = na;
-                    p.flags = fflags;
-                    p.tf0 = 0;
-                    hipGraphExec_t ex = nullptr;
-                    PSV_TRY(capture_steps(p, psv::kForward, +1, main_, main2_, fwd_tab_, true, &ex));
-                    graphs.push_back(ex);
-                    for (; end - it >= graph_steps_; it += graph_steps_) {
-                        PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
-                        PSV_HIP(hipGraphLaunch(ex, stream_));
-                        launches_ += graph_steps_;
+        AllSlotsAfter restore{this};
+        for (int w0 = 0; w0 < nslots_; w0 += group_size()) {
+            set_group(w0);
+            const int ntg = group_nt();
+            for (int it = 0; it < ntg; it++) {
+                const int na = active(it);
+                if (graph_steps_ > 0 && !snap && !ckpt) {
+                    // a run of steps with the same active shots: replay a captured
+                    // chunk of graph_steps_ steps, the step index read from it_dev_
+                    int end = it;
+                    while (end < ntg && active(end) == na) end++;
+                    if (end - it >= graph_steps_) {
+                        psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
+                        p.nslots = na;
+                        p.flags = fflags;
+                        p.tf0 = 0;
+                        hipGraphExec_t ex = nullptr;
+                        PSV_TRY(capture_steps(p, psv::kForward, +1, main_, main2_, fwd_tab_, true, &ex));
+                        graphs.push_back(ex);
+                        for (; end - it >= graph_steps_; it += graph_steps_) {
+                            PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
+                            PSV_HIP(hipGraphLaunch(ex, stream_));
+                            launches_ += graph_steps_;
+                        }
+                        if (it >= ntg) break;
                     }
-                    if (it >= ntmax_) break;
+                }
+                if (ckpt && it % kint == 0) PSV_TRY(save_checkpoint(it / kint, na));
+                psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
+                p.it = it;
+                p.nslots = active(it);
+                p.flags = fflags;
+                p.tf0 = 0;
+                PSV_TRY(launch_timed(p, psv::kForward, 0, main_, main2_));
+                if (snap && it % snap_interval_ == 0) {
+                    // forward_kernel_PSV_GPU.cu:1067-1080: after the step, every snap_interval
+                    const size_t k = (size_t)(it / snap_interval_);
+                    PSV_HIP(hipMemcpyAsync(snaps_.p + (k * nslots_ + w0_) * fst_, offw(main_.f[snap_field_], fst_),
+                                           (size_t)na * fst_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
                 }
             }
-            if (ckpt && it % kint == 0) PSV_TRY(save_checkpoint(it / kint, na));
-            psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
-            p.it = it;
-            p.nslots = active(it);
-            p.flags = fflags;
-            p.tf0 = 0;
-            PSV_TRY(launch_timed(p, psv::kForward, 0, main_, main2_));
-            if (snap && it % snap_interval_ == 0) {
-                // forward_kernel_PSV_GPU.cu:1067-1080: after the step, every snap_interval
-                const size_t k = (size_t)(it / snap_interval_);
-                PSV_HIP(hipMemcpyAsync(snaps_.p + k * nslots_ * fst_, main_.f[snap_field_].p,
-                                       (size_t)na * fst_ * sizeof(T), hipMemcpyDeviceToDevice, stream_));
-            }
         }
+        all_slots();
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         for (hipGraphExec_t ex : graphs) PSV_HIP(hipGraphExecDestroy(ex));
@@ -712,42 +760,48 @@ public:
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
         std::vector<hipGraphExec_t> graphs;
-        if (!ckpt) {
-            for (int it = ntmax_ - 1; it >= 0; it--) {
-                if (graph_steps_ > 0) {
-                    const int na = active(it);
-                    int end = it;  // steps it, it-1, ..., end+1 have the same active shots
-                    while (end >= 0 && active(end) == na) end--;
-                    if (it - end >= graph_steps_) {
-                        psv::StepParams<T> p = base_params(main_, adj_tab_, false);
-                        p.nslots = na;
-                        p.flags = aflags;
-                        p.tf0 = 0;
-                        hipGraphExec_t ex = nullptr;
-                        PSV_TRY(capture_steps(p, psv::kAdjoint, -1, main_, main2_, adj_tab_, false, &ex));
-                        graphs.push_back(ex);
-                        for (; it - end >= graph_steps_; it -= graph_steps_) {
-                            PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
-                            PSV_HIP(hipGraphLaunch(ex, stream_));
-                            launches_ += graph_steps_;
-                        }
-                        if (it < 0) break;
-                    }
-                }
-                PSV_TRY(adjoint_step(it, aflags, 0));
-            }
-        } else {
+        AllSlotsAfter restore{this};
+        if (ckpt) {
             PSV_TRY(alloc_state(replay_));
             if (fused()) {
                 PSV_TRY(alloc_state(replay2_));
                 PSV_TRY(zero_state(replay2_));
             }
+            PSV_TRY(alloc_store((ck_used_k_ + w_.dt - 1) / w_.dt + 1));
+        }
+        for (int w0 = 0; w0 < nslots_; w0 += group_size()) {
+            set_group(w0);
+            const int ntg = group_nt();
+            if (!ckpt) {
+                for (int it = ntg - 1; it >= 0; it--) {
+                    if (graph_steps_ > 0) {
+                        const int na = active(it);
+                        int end = it;  // steps it, it-1, ..., end+1 have the same active shots
+                        while (end >= 0 && active(end) == na) end--;
+                        if (it - end >= graph_steps_) {
+                            psv::StepParams<T> p = base_params(main_, adj_tab_, false);
+                            p.nslots = na;
+                            p.flags = aflags;
+                            p.tf0 = 0;
+                            hipGraphExec_t ex = nullptr;
+                            PSV_TRY(capture_steps(p, psv::kAdjoint, -1, main_, main2_, adj_tab_, false, &ex));
+                            graphs.push_back(ex);
+                            for (; it - end >= graph_steps_; it -= graph_steps_) {
+                                PSV_HIP(psv::launch_fill<int>(it_dev_.p, 1, it, stream_));
+                                PSV_HIP(hipGraphLaunch(ex, stream_));
+                                launches_ += graph_steps_;
+                            }
+                            if (it < 0) break;
+                        }
+                    }
+                    PSV_TRY(adjoint_step(it, aflags, 0));
+                }
+                continue;
+            }
             const int kk = ck_used_k_;
-            const int nseg = (ntmax_ + kk - 1) / kk;
-            const int seg_cap = (kk + w_.dt - 1) / w_.dt + 1;
-            PSV_TRY(alloc_store(seg_cap));
+            const int nseg = (ntg + kk - 1) / kk;
             for (int c = nseg - 1; c >= 0; c--) {
-                const int t0 = c * kk, t1 = std::min(t0 + kk, ntmax_);
+                const int t0 = c * kk, t1 = std::min(t0 + kk, ntg);
                 const int tf0 = (t0 + w_.dt - 1) / w_.dt;
                 // replay the segment forward from its checkpoint into the segment store
                 PSV_TRY(restore_checkpoint(c, active(t0)));
@@ -763,6 +817,7 @@ public:
                 for (int it = t1 - 1; it >= t0; it--) PSV_TRY(adjoint_step(it, aflags, tf0));
             }
         }
+        all_slots();
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         for (hipGraphExec_t ex : graphs) PSV_HIP(hipGraphExecDestroy(ex));
@@ -843,25 +898,30 @@ public:
         PSV_TRY(begin_samples());
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
-        for (int it = ntmax_ - 1; it >= 0; it--) {
-            const int na = active(it);
-            psv::StepParams<T> r = base_params(replay_, fwd_tab_noreceivers_, true);
-            r.it = it;
-            r.nslots = na;
-            r.flags = 0;
-            r.We = nullptr;
-            r.dt = -r.dt;
-            r.sig = neg_sig_.p;
-            PSV_HIP(psv::launch_stress_sources<T>(r, stream_));
-            PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 1, stream_));
-            PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 0, stream_));
-            if (it % w_.dt == 0) {
-                psv::StepParams<T> q = base_params(replay_, fwd_tab_noreceivers_, true);
-                q.nslots = na;
-                PSV_HIP(psv::launch_store_plane<T>(q, stream_));
+        AllSlotsAfter restore{this};
+        for (int w0 = 0; w0 < nslots_; w0 += group_size()) {
+            set_group(w0);
+            for (int it = group_nt() - 1; it >= 0; it--) {
+                const int na = active(it);
+                psv::StepParams<T> r = base_params(replay_, fwd_tab_noreceivers_, true);
+                r.it = it;
+                r.nslots = na;
+                r.flags = 0;
+                r.We = nullptr;
+                r.dt = -r.dt;
+                r.sig = neg_sig_.p;
+                PSV_HIP(psv::launch_stress_sources<T>(r, stream_));
+                PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 1, stream_));
+                PSV_HIP(psv::launch_half<T>(r, g_.fdorder, psv::kForward, tz_, 0, stream_));
+                if (it % w_.dt == 0) {
+                    psv::StepParams<T> q = base_params(replay_, fwd_tab_noreceivers_, true);
+                    q.nslots = na;
+                    PSV_HIP(psv::launch_store_plane<T>(q, stream_));
+                }
+                PSV_TRY(adjoint_step(it, aflags, it / w_.dt));
             }
-            PSV_TRY(adjoint_step(it, aflags, it / w_.dt));
         }
+        all_slots();
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         float ms = 0;
@@ -916,28 +976,34 @@ public:
 
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
-        const int nseg = (ntmax_ + kk - 1) / kk;
-        for (int c = nseg - 1; c >= 0; c--) {
-            const int t0 = c * kk, t1 = std::min(t0 + kk, ntmax_);
-            PSV_TRY(restore_checkpoint(c, active(t0)));
-            PSV_TRY(state_copy<true>(seg_.p, active(t0), replay_));
-            for (int it = t0; it < t1; it++) {
-                psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
-                p.it = it;
-                p.nslots = active(it);
-                p.flags = 0;
-                p.We = nullptr;
-                PSV_TRY(launch_plain(p, psv::kForward, replay_, replay2_));
-                PSV_TRY(state_copy<true>(seg_.p + (size_t)(it - t0 + 1) * state_elems(), active(it), replay_));
-                launches_++;
-            }
-            for (int it = t1 - 1; it >= t0; it--) {
-                psv::DiscreteParams<T> d = dparams(it, entry(it - t0), entry(it - t0 + 1));
-                d.max_groups = maxg;
-                PSV_HIP(psv::launch_discrete_step<T>(d, g_.fdorder, stream_));
-                launches_++;
+        AllSlotsAfter restore{this};
+        for (int w0 = 0; w0 < nslots_; w0 += group_size()) {
+            set_group(w0);
+            const int ntg = group_nt();
+            const int nseg = (ntg + kk - 1) / kk;
+            for (int c = nseg - 1; c >= 0; c--) {
+                const int t0 = c * kk, t1 = std::min(t0 + kk, ntg);
+                PSV_TRY(restore_checkpoint(c, active(t0)));
+                PSV_TRY(state_copy<true>(seg_.p, active(t0), replay_));
+                for (int it = t0; it < t1; it++) {
+                    psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = 0;
+                    p.We = nullptr;
+                    PSV_TRY(launch_plain(p, psv::kForward, replay_, replay2_));
+                    PSV_TRY(state_copy<true>(seg_.p + (size_t)(it - t0 + 1) * state_elems(), active(it), replay_));
+                    launches_++;
+                }
+                for (int it = t1 - 1; it >= t0; it--) {
+                    psv::DiscreteParams<T> d = dparams(it, entry(it - t0), entry(it - t0 + 1));
+                    d.max_groups = maxg;
+                    PSV_HIP(psv::launch_discrete_step<T>(d, g_.fdorder, stream_));
+                    launches_++;
+                }
             }
         }
+        all_slots();
         PSV_HIP(hipEventRecord(ev1_, stream_));
         PSV_HIP(hipStreamSynchronize(stream_));
         float ms = 0;
@@ -960,18 +1026,19 @@ public:
         const T* base = seg_.p + (size_t)e * state_elems();
         View v{};
         size_t off = 0;
+        const size_t w0 = (size_t)w0_;
         for (int f = 0; f < 5; f++) {
-            v.f[f] = base + off + org_;
+            v.f[f] = base + off + org_ + w0 * fst_;
             off += (size_t)nslots_ * fst_;
         }
         if (g_.npml > 0) {
             const size_t st[4] = {xst_, xst_, zst_, zst_};
             for (int m = 0; m < 4; m++) {
-                v.ms[m] = base + off;
+                v.ms[m] = base + off + w0 * st[m];
                 off += (size_t)nslots_ * st[m];
             }
             for (int m = 0; m < 4; m++) {
-                v.mv[m] = base + off;
+                v.mv[m] = base + off + w0 * st[m];
                 off += (size_t)nslots_ * st[m];
             }
         }
@@ -1019,34 +1086,34 @@ public:
         d.post_sxx = post.f[kF_SXX];
         d.post_szx = post.f[kF_SZX];
         d.post_szz = post.f[kF_SZZ];
-        d.vbx = at(adj_.f[kF_VX]);
-        d.vbz = at(adj_.f[kF_VZ]);
-        d.sbxx = at(adj_.f[kF_SXX]);
-        d.sbzx = at(adj_.f[kF_SZX]);
-        d.sbzz = at(adj_.f[kF_SZZ]);
-        d.bvxx = adj_.ms[0].p;
-        d.bvzx = adj_.ms[1].p;
-        d.bvxz = adj_.ms[2].p;
-        d.bvzz = adj_.ms[3].p;
-        d.bsxxx = adj_.mv[0].p;
-        d.bszxx = adj_.mv[1].p;
-        d.bszzz = adj_.mv[2].p;
-        d.bszxz = adj_.mv[3].p;
-        d.w0 = at(dw_[0]);
-        d.w1 = at(dw_[1]);
-        d.w2 = at(dw_[2]);
-        d.w3 = at(dw_[3]);
-        d.g_lam = at(dg_[0]);
-        d.g_mu = at(dg_[1]);
-        d.g_mu_zx = at(dg_[2]);
-        d.g_rho_zp = at(dg_[3]);
-        d.g_rho_xp = at(dg_[4]);
+        d.vbx = atw(adj_.f[kF_VX], fst_);
+        d.vbz = atw(adj_.f[kF_VZ], fst_);
+        d.sbxx = atw(adj_.f[kF_SXX], fst_);
+        d.sbzx = atw(adj_.f[kF_SZX], fst_);
+        d.sbzz = atw(adj_.f[kF_SZZ], fst_);
+        d.bvxx = offw(adj_.ms[0], xst_);
+        d.bvzx = offw(adj_.ms[1], xst_);
+        d.bvxz = offw(adj_.ms[2], zst_);
+        d.bvzz = offw(adj_.ms[3], zst_);
+        d.bsxxx = offw(adj_.mv[0], xst_);
+        d.bszxx = offw(adj_.mv[1], xst_);
+        d.bszzz = offw(adj_.mv[2], zst_);
+        d.bszxz = offw(adj_.mv[3], zst_);
+        d.w0 = atw(dw_[0], fst_);
+        d.w1 = atw(dw_[1], fst_);
+        d.w2 = atw(dw_[2], fst_);
+        d.w3 = atw(dw_[3], fst_);
+        d.g_lam = atw(dg_[0], fst_);
+        d.g_mu = atw(dg_[1], fst_);
+        d.g_mu_zx = atw(dg_[2], fst_);
+        d.g_rho_zp = atw(dg_[3], fst_);
+        d.g_rho_xp = atw(dg_[4], fst_);
         d.grp = adj_tab_.inj_grp.p;
-        d.grp_off = dgrp_off_.p;
+        d.grp_off = plus(dgrp_off_.p, w0_);
         d.grp_row = adj_tab_.inj_row.p;
         d.sig = adj_sig_.p;
-        d.sig_off = rec_out_off_.p;
-        d.shot_nt = shot_nt_.p;
+        d.sig_off = plus(rec_out_off_.p, w0_);
+        d.shot_nt = plus(shot_nt_.p, w0_);
         return d;
     }
 
@@ -1251,12 +1318,53 @@ private:
 
     bool in_grid(int z, int x) const { return z >= 0 && z < g_.nzt && x >= 0 && x < g_.nxt; }
 
+    // shots of the current launch group [w0_, w1_) still running at step it
+    // (slots are sorted by decreasing nt, so they are a prefix of the group)
     int active(int it) const
     {
         int n = 0;
-        while (n < nslots_ && nt_slot_[(size_t)n] > it) n++;
+        while (w0_ + n < w1_ && nt_slot_[(size_t)(w0_ + n)] > it) n++;
         return n;
     }
+
+    // Launch groups: the time loops of a batch run group by group, lg_
+    // consecutive slots at a time (0: the whole batch), so that one group's
+    // wavefields stay in the 256 MB Infinity Cache from one step to the next.
+    // The fused step swaps whole buffer sets after each step, so it runs the
+    // batch as one group.
+    int group_size() const { return (lg_ > 0 && !fused()) ? std::min(lg_, nslots_) : nslots_; }
+
+    void set_group(int w0)
+    {
+        w0_ = w0;
+        w1_ = std::min(nslots_, w0 + group_size());
+    }
+
+    void all_slots()
+    {
+        w0_ = 0;
+        w1_ = nslots_;
+    }
+
+    // restores the whole batch as the current group when a run ends, also on
+    // an early error return
+    struct AllSlotsAfter {
+        Plan* q;
+        ~AllSlotsAfter() { q->all_slots(); }
+    };
+
+    // length of the current group's time loop
+    int group_nt() const { return nt_slot_[(size_t)w0_]; }
+
+    template <typename U>
+    static U* plus(U* q, size_t o)
+    {
+        return q ? q + o : nullptr;
+    }
+
+    // device pointer to the current group's first slot of a per-slot array
+    T* atw(DBuf<T>& b, size_t stride) const { return b.p ? b.p + org_ + (size_t)w0_ * stride : nullptr; }
+    T* offw(DBuf<T>& b, size_t stride) const { return b.p ? b.p + (size_t)w0_ * stride : nullptr; }
 
     psvfwi_status ready() const
     {
@@ -1514,16 +1622,18 @@ private:
         return state_copy<SAVE>(ckpt_.p + (size_t)c * state_elems(), na, st);
     }
 
-    // copy/restore the first na slots of a saved state at base: each array is
-    // [nslots][stride], so a prefix of slots is one contiguous range
+    // copy/restore the first na slots of the current launch group of a saved
+    // state at base: each array is [nslots][stride], so a run of slots is one
+    // contiguous range
     template <bool SAVE>
     psvfwi_status state_copy(T* base, int na, State<T>& st)
     {
         // one launch for all 13 arrays (sizes and offsets are multiples of 32 elements)
         psv::CopyList l{};
         size_t off = 0;
-        auto one = [&](T* arr, size_t stride) {
-            T* ck = base + off;
+        auto one = [&](T* arr0, size_t stride) {
+            T* ck = base + off + (size_t)w0_ * stride;
+            T* arr = arr0 + (size_t)w0_ * stride;
             l.src[l.n] = SAVE ? (const void*)arr : (const void*)ck;
             l.dst[l.n] = SAVE ? (void*)ck : (void*)arr;
             l.bytes[l.n] = (size_t)na * stride * sizeof(T);
@@ -1674,35 +1784,35 @@ private:
         p.rho_zp = at(rho_zp_);
         p.rho_xp = at(rho_xp_);
         p.pml = pml_.p;
-        p.vx = at(st.f[kF_VX]);
-        p.vz = at(st.f[kF_VZ]);
-        p.sxx = at(st.f[kF_SXX]);
-        p.szx = at(st.f[kF_SZX]);
-        p.szz = at(st.f[kF_SZZ]);
+        p.vx = atw(st.f[kF_VX], fst_);
+        p.vz = atw(st.f[kF_VZ], fst_);
+        p.sxx = atw(st.f[kF_SXX], fst_);
+        p.szx = atw(st.f[kF_SZX], fst_);
+        p.szz = atw(st.f[kF_SZZ], fst_);
         p.vx_w = p.vx;
         p.vz_w = p.vz;
         p.sxx_w = p.sxx;
         p.szx_w = p.szx;
         p.szz_w = p.szz;
         p.fst = fst_;
-        p.We = at(We_);
-        p.ux = at(ux_);
-        p.uz = at(uz_);
-        p.mvxx = st.ms[0].p;
-        p.mvzx = st.ms[1].p;
-        p.mvxz = st.ms[2].p;
-        p.mvzz = st.ms[3].p;
+        p.We = atw(We_, fst_);
+        p.ux = atw(ux_, fst_);
+        p.uz = atw(uz_, fst_);
+        p.mvxx = offw(st.ms[0], xst_);
+        p.mvzx = offw(st.ms[1], xst_);
+        p.mvxz = offw(st.ms[2], zst_);
+        p.mvzz = offw(st.ms[3], zst_);
         p.mvxx_w = p.mvxx;
         p.mvzx_w = p.mvzx;
         p.mvxz_w = p.mvxz;
         p.mvzz_w = p.mvzz;
-        p.msxxx = st.mv[0].p;
-        p.mszxx = st.mv[1].p;
-        p.mszzz = st.mv[2].p;
-        p.mszxz = st.mv[3].p;
+        p.msxxx = offw(st.mv[0], xst_);
+        p.mszxx = offw(st.mv[1], xst_);
+        p.mszzz = offw(st.mv[2], zst_);
+        p.mszxz = offw(st.mv[3], zst_);
         p.xst = xst_;
         p.zst = zst_;
-        p.store = store_.p;
+        p.store = offw(store_, sst_);
         p.sst = sst_;
         if (win_on_) {
             p.fwi_dt = w_.dt;
@@ -1717,28 +1827,30 @@ private:
         } else {
             p.fwi_dt = p.fwi_dx = p.fwi_dz = 1;
         }
-        p.grad = grad_.p;
+        p.grad = offw(grad_, gst_);
         p.gst = gst_;
-        p.shot_nt = shot_nt_.p;
+        p.shot_nt = plus(shot_nt_.p, w0_);
         p.inj_grp = tab.inj_grp.p;
         p.inj_row = tab.inj_row.p;
-        p.tinj_off = tab.tinj_off.p;
+        const size_t ntile = (size_t)p.tiles_x * p.tiles_z;
+        p.tinj_off = plus(tab.tinj_off.p, (size_t)w0_ * ntile);
         p.tinj_idx = tab.tinj_idx.p;
-        p.trec_off = tab.trec_off.p;
+        p.trec_off = plus(tab.trec_off.p, (size_t)w0_ * ntile);
         p.trec_idx = tab.trec_idx.p;
         if (fwd_sig) {
             p.sig = fwd_tab_.sig.p;
-            p.sig_off = fwd_tab_.sig_off.p;
+            p.sig_off = plus(fwd_tab_.sig_off.p, w0_);
         } else {
             p.sig = adj_sig_.p;
-            p.sig_off = rec_out_off_.p;
+            p.sig_off = plus(rec_out_off_.p, w0_);
         }
-        p.rec_first = tab.rec_first.p;
+        p.rec_first = plus(tab.rec_first.p, w0_);
         p.rec_zxc = tab.rec_zxc.p;
         p.rec = rec_.p;
-        p.rec_out_off = rec_out_off_.p;
+        p.rec_out_off = plus(rec_out_off_.p, w0_);
         p.nexpl = fwd_sig ? nexpl_ : 0;
-        p.expl_grp = expl_grp_.p;
+        p.expl_grp = expl_grp_.p;  // (group, slot) pairs of the whole batch
+        p.slot0 = w0_;
         return p;
     }
 
@@ -1747,6 +1859,8 @@ private:
     bool parity_ = false;
     int graph_steps_ = 0;  // > 0: time loops replay captured chunks of this many steps
     int band_ = 0;         // block order of a batch (StepParams::band)
+    int lg_ = 0;           // shots per launch group (0: the whole batch)
+    int w0_ = 0, w1_ = 0;  // the current launch group: slots [w0_, w1_)
     DBuf<int> it_dev_;
     size_t fst_ = 0, xst_ = 0, zst_ = 0;
     hipStream_t stream_ = nullptr;
@@ -1930,6 +2044,18 @@ psvfwi_status psvfwi_get_shot_order(const psvfwi_plan* plan, int* band)
 {
     PSV_PLAN(plan);
     return plan->shot_order(band);
+}
+
+psvfwi_status psvfwi_set_launch_group(psvfwi_plan* plan, int shots)
+{
+    PSV_PLAN(plan);
+    return plan->set_launch_group(shots);
+}
+
+psvfwi_status psvfwi_get_launch_group(const psvfwi_plan* plan, int* shots)
+{
+    PSV_PLAN(plan);
+    return plan->launch_group(shots);
 }
 
 psvfwi_status psvfwi_set_snapshots(psvfwi_plan* plan, int interval, int field)

@@ -37,7 +37,7 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps",
            "psvfwi_set_gradient_mode", "psvfwi_get_coefficient_gradient", "psvfwi_get_model_gradient",
            "psvfwi_chain_gradient", "psvfwi_random_boundary", "psvfwi_set_shot_order",
-           "psvfwi_get_shot_order")
+           "psvfwi_get_shot_order", "psvfwi_set_launch_group", "psvfwi_get_launch_group")
 
 
 class PsvfwiError(RuntimeError):
@@ -180,6 +180,15 @@ class Plan:
         b = C.c_int()
         _check(lib().psvfwi_get_shot_order(self._h, C.byref(b)))
         return b.value
+
+    def set_launch_group(self, shots):
+        """Run the batch `shots` slots at a time, every step of a group before the next (0: all)."""
+        _check(lib().psvfwi_set_launch_group(self._h, int(shots)))
+
+    def launch_group(self):
+        k = C.c_int()
+        _check(lib().psvfwi_get_launch_group(self._h, C.byref(k)))
+        return k.value
 
     def set_graph_steps(self, steps):
         """Replay time steps as captured HIP graphs of `steps` launches (0: off)."""

@@ -125,7 +125,8 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
 /* Autotuner: times `steps` forward steps of the current batch with every
  * step kernel and tile height of psvfwi_set_tile, keeps the fastest and
  * returns its psvfwi_set_tile value in *best_tile (may be NULL); for a batch it
- * then chooses the shot order (psvfwi_set_shot_order) the same way.  In the
+ * then chooses the shot order (psvfwi_set_shot_order) and, with the split
+ * step, the launch group (psvfwi_set_launch_group) the same way.  In the
  * random-boundary store mode only the split kernels (tz > 0) are tried.
  * Needs set_model / set_pml / set_shots first.  (No counterpart in the
  * reference, whose launch shape is fixed at 16 x 16,
@@ -138,6 +139,16 @@ psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile);
  * chooses among 0, 1 and 4 for batches of more than one shot.  Results do not depend on it. */
 psvfwi_status psvfwi_set_shot_order(psvfwi_plan* plan, int band);
 psvfwi_status psvfwi_get_shot_order(const psvfwi_plan* plan, int* band);
+
+/* Launch groups (no counterpart in the reference, which runs one shot per launch): with
+ * shots > 0 the time loops of a batch run `shots` slots at a time, all steps of one group
+ * before the next (slots are the shots sorted by decreasing nt), so that a group's
+ * wavefields stay in the GPU's 256 MB Infinity Cache from one step to the next; 0 (default)
+ * steps the whole batch together.  The fused step (tile < 0) always runs the batch as one
+ * group.  psvfwi_autotune chooses among 0, 16, 12, 8, 6, 4 and 2 for batches.  Results do not
+ * depend on it. */
+psvfwi_status psvfwi_set_launch_group(psvfwi_plan* plan, int shots);
+psvfwi_status psvfwi_get_launch_group(const psvfwi_plan* plan, int* shots);
 
 /* Graph-captured time loop: with steps > 0 (even), forward and adjoint
  * runs replay HIP graphs of `steps` consecutive step launches (the step index
