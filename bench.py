@@ -376,11 +376,33 @@ def run_c5(args):
     inv = Inversion(plan, shots, observed, start, win, segment=args.interval, mute=args.c5_mute,
                     group=args.c5_group or None)
     err0 = inv.model_error(truth)
+    # a run longer than one job continues from the state a previous job saved:
+    # the FP64 model over the window (the padding is edge replication of it),
+    # the last step length and the per-iteration history
+    prev = {"misfit": [], "step": [], "trials": [], "seconds": [], "model_error": []}
+    if args.c5_resume:
+        st = np.load(args.c5_resume, allow_pickle=False)
+        z1, z2, x1, x2 = win
+        for k in ("lam", "mu", "rho"):
+            m = inv.m[k].copy()
+            m[z1:z2, x1:x2] = st[k]
+            inv.m[k] = np.pad(m[z1:z2, x1:x2], ((z1, m.shape[0] - z2), (x1, m.shape[1] - x2)), mode="edge")
+        inv.step = float(st["step"])
+        prev = json.loads(str(st["history"]))
+        _log(f"resumed after {len(prev['misfit'])} iterations, step {inv.step:.4g}, "
+             f"vp error {inv.model_error(truth):.5f}")
     _sync()
     t0 = time.perf_counter()
-    h = inv.run(args.iters, truth=truth, log=_log)
+    h = inv.run(args.iters, truth=truth, log=_log, first=len(prev["misfit"]))
     _sync()
     T = time.perf_counter() - t0
+    for k in prev:
+        prev[k] = list(prev[k]) + [float(x) for x in getattr(h, k)]
+    if args.c5_save:
+        z1, z2, x1, x2 = win
+        np.savez(args.c5_save, lam=inv.m["lam"][z1:z2, x1:x2], mu=inv.m["mu"][z1:z2, x1:x2],
+                 rho=inv.m["rho"][z1:z2, x1:x2], step=np.float64(inv.step), history=json.dumps(prev))
+        _log(f"state after {len(prev['misfit'])} iterations saved to {args.c5_save}")
     cells = w.interior_cells()
     upd = 2.0 * cells * nt * nshots * args.iters
     return {
@@ -395,8 +417,9 @@ def run_c5(args):
                                f"{args.interval} steps, top {args.c5_mute} model rows not updated",
                    "scale_of_2400x800_128_shots": sc, "tile": plan.tile,
                    "launch_group": plan.launch_group() or "whole batch"},
-        "fwi": {"misfit": h.misfit, "vp_rel_error_start": err0, "vp_rel_error": h.model_error,
-                "step": h.step, "trial_forwards": h.trials, "seconds": [round(x, 2) for x in h.seconds],
+        "fwi": {"misfit": prev["misfit"], "vp_rel_error_start": err0, "vp_rel_error": prev["model_error"],
+                "step": prev["step"], "trial_forwards": prev["trials"],
+                "seconds": [round(x, 2) for x in prev["seconds"]], "iterations_this_run": args.iters,
                 "note": "value counts forward + adjoint cell updates of each iteration; trial forwards "
                         "and replays are not counted"},
     }
@@ -620,6 +643,9 @@ def main():
                          "shots does not fit in HBM at once")
     ap.add_argument("--c5-mute", type=int, default=10,
                     help="c5: model rows below the surface with no update (sources and receivers at row 2)")
+    ap.add_argument("--c5-resume", default="",
+                    help="c5: continue from a state file written by --c5-save")
+    ap.add_argument("--c5-save", default="", help="c5: write the model, step and history here after the run")
     ap.add_argument("--c5-scale", type=float, default=0.5,
                     help="c5: lengths, shot and receiver counts and nt relative to 2400 x 800 / 128 / 600 / 6000")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])

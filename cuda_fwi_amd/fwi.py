@@ -184,7 +184,8 @@ class Inversion:
         self.step = opt
         return L2, opt, ntrial
 
-    def run(self, iters, truth=None, log=None):
+    def run(self, iters, truth=None, log=None, first=0):
+        """iters iterations; first numbers the first one in the log lines."""
         self.log = log
         h = History()
         for i in range(iters):
@@ -197,7 +198,7 @@ class Inversion:
             if truth is not None:
                 h.model_error.append(self.model_error(truth))
             if log:
-                log(f"iteration {i}: misfit {L2:.6e}, step {s:.4g}, {n} trials, {h.seconds[-1]:.1f} s"
+                log(f"iteration {first + i}: misfit {L2:.6e}, step {s:.4g}, {n} trials, {h.seconds[-1]:.1f} s"
                     + (f", vp error {h.model_error[-1]:.4f}" if truth is not None else ""))
         return h
 
