@@ -35,8 +35,12 @@ struct DiscreteParams {
     T *vbx, *vbz, *sbxx, *sbzx, *sbzz;
     T *bvxx, *bvzx, *bvxz, *bvzz;
     T *bsxxx, *bszxx, *bszzz, *bszxz;
-    // weights of the derivative terms (full-grid layout, zero outside the interior)
+    // weights of the derivative terms (full-grid layout, zero outside the
+    // interior): w* of the velocity half-step, ws* of the stress half-step
+    // (separate arrays: the stress pass reads the velocity weights of
+    // neighbouring cells while it writes its own cell's stress weights)
     T *w0, *w1, *w2, *w3;
+    T *ws0, *ws1, *ws2, *ws3;
     // coefficient gradients (full-grid layout)
     T *g_lam, *g_mu, *g_mu_zx, *g_rho_zp, *g_rho_xp;
     // receiver adjoint sources: groups of co-located receivers per slot

@@ -3,13 +3,16 @@
 // The transpose of one forward step (psv_step.hip, the reference's
 // forward_kernel_PSV_GPU.cu:830-1063) applied in reverse order:
 //   receivers -> velocity half-step -> stress mirror -> stress half-step.
-// Each half-step is two launches: a weight kernel, one thread per interior
-// cell, that recomputes the forward derivative terms of the cell, adds the
-// cell's gradient contributions and forms the adjoint weights of the four
-// derivative terms (through the transposed CPML recursions); then a gather
-// kernel that applies the transposed difference stencils to those weights.
+// Each half-step has a weight pass, one thread per interior cell, that
+// recomputes the forward derivative terms of the cell, adds the cell's
+// gradient contributions and forms the adjoint weights of the four
+// derivative terms (through the transposed CPML recursions), and a gather
+// pass that applies the transposed difference stencils to those weights.
 // Splitting at the weights keeps every adjoint update a gather (no
-// scatter, no atomics) and the order of every sum fixed.
+// scatter, no atomics) and the order of every sum fixed.  A reverse step is
+// three launches after the receivers: velocity weights; stress gather,
+// transposed mirror and stress weights (the stress weights need the stress
+// adjoints of their own cell only); velocity gather.
 //
 // The per-cell expressions are fixed: the CPU restatement evaluates the
 // same ones in the same order.
@@ -168,53 +171,43 @@ __global__ __launch_bounds__(kDX* kDZ) void d_vel_weights_kernel(const DiscreteP
     p.w3[k] = w_szz_z;
 }
 
-// stress adjoints += transposed stress differences of the weights (every
-// cell of the grid: the ghost rows above a free surface feed the mirror)
+// increments of the stress adjoints at offset k: the transposed stress
+// differences of the velocity half-step's weights
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kDX* kDZ) void d_vel_gather_kernel(const DiscreteParams<T> p)
+__device__ __forceinline__ void vel_gather_inc(const DiscreteParams<T>& p, size_t k, T& ixx, T& izx, T& izz)
 {
-    const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
-    if (iz >= p.nzt || ix >= p.nxt) return;
-    const size_t k = (size_t)blockIdx.z * p.fst + (long)iz * p.pitch + ix;
     const Cell<T> W0{p.w0 + k, p.pitch}, W1{p.w1 + k, p.pitch}, W2{p.w2 + k, p.pitch},
         W3{p.w3 + k, p.pitch};
     const T dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
     if (ORDER == 2) {
-        p.sbxx[k] = p.sbxx[k] + dxi * hc1 * (W0(0, -1) - W0(0, 0));
+        ixx = dxi * hc1 * (W0(0, -1) - W0(0, 0));
         const T tx = dxi * hc1 * (W1(0, 0) - W1(0, 1));
         const T tz = dzi * hc1 * (W2(0, 0) - W2(1, 0));
-        p.sbzx[k] = p.sbzx[k] + (tx + tz);
-        p.sbzz[k] = p.sbzz[k] + dzi * hc1 * (W3(-1, 0) - W3(0, 0));
+        izx = tx + tz;
+        izz = dzi * hc1 * (W3(-1, 0) - W3(0, 0));
     } else {
-        p.sbxx[k] = p.sbxx[k] + dxi * (hc1 * (W0(0, -1) - W0(0, 0)) + hc2 * (W0(0, -2) - W0(0, 1)));
+        ixx = dxi * (hc1 * (W0(0, -1) - W0(0, 0)) + hc2 * (W0(0, -2) - W0(0, 1)));
         const T tx = dxi * (hc1 * (W1(0, 0) - W1(0, 1)) + hc2 * (W1(0, -1) - W1(0, 2)));
         const T tz = dzi * (hc1 * (W2(0, 0) - W2(1, 0)) + hc2 * (W2(-1, 0) - W2(2, 0)));
-        p.sbzx[k] = p.sbzx[k] + (tx + tz);
-        p.sbzz[k] = p.sbzz[k] + dzi * (hc1 * (W3(-1, 0) - W3(0, 0)) + hc2 * (W3(-2, 0) - W3(1, 0)));
+        izx = tx + tz;
+        izz = dzi * (hc1 * (W3(-1, 0) - W3(0, 0)) + hc2 * (W3(-2, 0) - W3(1, 0)));
     }
 }
 
-// transposed mirror s[fpad - k] = -s[fpad + k] of szx and szz
+// The stress side of a reverse step in one pass over the interior cells:
+//   1. stress adjoints += the transposed stress differences of the velocity
+//      half-step's weights;
+//   2. with a free surface, the transposed mirror s[fpad - k] = -s[fpad + k]
+//      of szx and szz: the rows that the ghost rows mirror receive minus the
+//      ghost rows' adjoints, which are recomputed here as 0 + their gathered
+//      increments (the ghost rows' adjoints are zero before every gather and
+//      are never stored);
+//   3. the transposed stress half-step: weights and the lam / mu / mu_zx
+//      gradients.
+// Every value is the one the three steps give as separate passes over the
+// whole grid, bit for bit.
 template <typename T, int ORDER>
-__global__ void d_mirror_kernel(const DiscreteParams<T> p)
-{
-    constexpr int H = ORDER / 2;
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix < H || ix >= p.nxt - H) return;
-    const size_t so = (size_t)blockIdx.y * p.fst;
-    for (int kk = 1; kk <= H; kk++) {
-        const size_t g = so + (long)(H - kk) * p.pitch + ix, o = so + (long)(H + kk) * p.pitch + ix;
-        const size_t ox = ORDER == 2 ? o : o - p.pitch;  // order 4: szx mirrors row H + kk - 1
-        p.sbzx[ox] = p.sbzx[ox] - p.sbzx[g];
-        p.sbzz[o] = p.sbzz[o] - p.sbzz[g];
-        p.sbzx[g] = (T)0;
-        p.sbzz[g] = (T)0;
-    }
-}
-
-// transposed stress half-step: weights and the lam / mu / mu_zx gradients
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kDX* kDZ) void d_stress_weights_kernel(const DiscreteParams<T> p)
+__global__ __launch_bounds__(kDX* kDZ) void d_stress_pass_kernel(const DiscreteParams<T> p)
 {
     constexpr int H = ORDER / 2;
     const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
@@ -283,7 +276,28 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_weights_kernel(const Discre
     }
     const size_t k = so + c;
     const T l = p.lam[c], m = p.mu[c];
-    const T bxx = p.sbxx[k], bzx = p.sbzx[k], bzz = p.sbzz[k];
+    T bxx, bzx, bzz;
+    {
+        T ixx, izx, izz;
+        vel_gather_inc<T, ORDER>(p, k, ixx, izx, izz);
+        bxx = p.sbxx[k] + ixx;
+        bzx = p.sbzx[k] + izx;
+        bzz = p.sbzz[k] + izz;
+    }
+    if (fsurf && iz <= 2 * H) {
+#pragma unroll
+        for (int kk = 1; kk <= H; kk++) {
+            // szz of ghost row H - kk goes to row H + kk; szx to row H + kk
+            // (order 2) or H + kk - 1 (order 4)
+            const bool to_zz = iz == H + kk, to_zx = iz == (ORDER == 2 ? H + kk : H + kk - 1);
+            if (to_zz || to_zx) {
+                T gxx, gzx, gzz;
+                vel_gather_inc<T, ORDER>(p, so + (long)(H - kk) * p.pitch + ix, gxx, gzx, gzz);
+                if (to_zz) bzz = bzz - ((T)0 + gzz);
+                if (to_zx) bzx = bzx - ((T)0 + gzx);
+            }
+        }
+    }
     T wxx, wzz, wzx;
     if (fsurf && iz == H) {
         // free surface, order 2: sxx = 4 dt vx_x (l m + m^2) / (l + 2 m), szz = szx = 0;
@@ -301,9 +315,14 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_weights_kernel(const Discre
         } else {
             wzx = dt * p.mu_zx[c] * bzx;
             p.g_mu_zx[k] = p.g_mu_zx[k] + bzx * (dt * (vz_x + vx_z));
+            p.sbxx[k] = bxx;
+            p.sbzx[k] = bzx;
         }
         p.sbzz[k] = (T)0;
     } else {
+        p.sbxx[k] = bxx;
+        p.sbzx[k] = bzx;
+        p.sbzz[k] = bzz;
         wxx = dt * (l * (bxx + bzz) + (T)2.0 * m * bxx);
         wzz = dt * (l * (bxx + bzz) + (T)2.0 * m * bzz);
         wzx = dt * p.mu_zx[c] * bzx;
@@ -338,10 +357,10 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_weights_kernel(const Discre
             w_vz_x = cpml_t(w_vz_x, &p.bvzx[q], pah[px], pbh[px], pKh[px]);
         }
     }
-    p.w0[k] = w_vx_x;
-    p.w1[k] = w_vz_x;
-    p.w2[k] = w_vx_z;
-    p.w3[k] = w_vz_z;
+    p.ws0[k] = w_vx_x;
+    p.ws1[k] = w_vz_x;
+    p.ws2[k] = w_vx_z;
+    p.ws3[k] = w_vz_z;
 }
 
 // velocity adjoints += transposed velocity differences of the weights
@@ -353,8 +372,8 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_gather_kernel(const Discret
     const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
     if (iz < H || iz >= p.nzt - H || ix < H || ix >= p.nxt - H) return;
     const size_t k = (size_t)blockIdx.z * p.fst + (long)iz * p.pitch + ix;
-    const Cell<T> W0{p.w0 + k, p.pitch}, W1{p.w1 + k, p.pitch}, W2{p.w2 + k, p.pitch},
-        W3{p.w3 + k, p.pitch};
+    const Cell<T> W0{p.ws0 + k, p.pitch}, W1{p.ws1 + k, p.pitch}, W2{p.ws2 + k, p.pitch},
+        W3{p.ws3 + k, p.pitch};
     const T dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
     if (ORDER == 2) {
         const T xx = dxi * hc1 * (W0(0, 0) - W0(0, 1));
@@ -382,11 +401,7 @@ hipError_t discrete_step(const DiscreteParams<T>& p, hipStream_t s)
         hipLaunchKernelGGL((d_inject_kernel<T>), dim3((p.max_groups + 63) / 64, p.nslots), dim3(64), 0,
                            s, p);
     hipLaunchKernelGGL((d_vel_weights_kernel<T, ORDER>), grd, blk, 0, s, p);
-    hipLaunchKernelGGL((d_vel_gather_kernel<T, ORDER>), grd, blk, 0, s, p);
-    if (p.fsurf)
-        hipLaunchKernelGGL((d_mirror_kernel<T, ORDER>), dim3((p.nxt + 63) / 64, p.nslots), dim3(64), 0,
-                           s, p);
-    hipLaunchKernelGGL((d_stress_weights_kernel<T, ORDER>), grd, blk, 0, s, p);
+    hipLaunchKernelGGL((d_stress_pass_kernel<T, ORDER>), grd, blk, 0, s, p);
     hipLaunchKernelGGL((d_stress_gather_kernel<T, ORDER>), grd, blk, 0, s, p);
     return hipGetLastError();
 }

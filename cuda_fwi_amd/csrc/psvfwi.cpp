@@ -953,7 +953,7 @@ public:
         PSV_TRY(alloc_state(adj_));
         PSV_TRY(zero_state(adj_));
         const size_t nf = (size_t)nslots_ * fst_;
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < 8; i++) {
             PSV_HIP(dw_[i].alloc(nf));
             PSV_HIP(psv::launch_fill<T>(dw_[i].p, nf, (T)0, stream_));
         }
@@ -1103,6 +1103,10 @@ public:
         d.w1 = atw(dw_[1], fst_);
         d.w2 = atw(dw_[2], fst_);
         d.w3 = atw(dw_[3], fst_);
+        d.ws0 = atw(dw_[4], fst_);
+        d.ws1 = atw(dw_[5], fst_);
+        d.ws2 = atw(dw_[6], fst_);
+        d.ws3 = atw(dw_[7], fst_);
         d.g_lam = atw(dg_[0], fst_);
         d.g_mu = atw(dg_[1], fst_);
         d.g_mu_zx = atw(dg_[2], fst_);
@@ -1909,7 +1913,7 @@ private:
     std::vector<T> fwd_sig_h_;
     DBuf<T> neg_sig_;
     State<T> adj_;
-    DBuf<T> dw_[4], dg_[5], seg_;
+    DBuf<T> dw_[8], dg_[5], seg_;  // dw_: velocity weights 0-3, stress weights 4-7
     DBuf<int> dgrp_off_;
     std::vector<double> h_pml_, h_mu_, h_mu_zx_, h_rho_zp_, h_rho_xp_;
     bool have_obs_ = false;

@@ -114,6 +114,45 @@ def test_discrete_gradient_batched_uneven_shots():
         assert np.allclose(tot[k], want, rtol=1e-13, atol=0), k
 
 
+def test_discrete_gradient_many_workgroups():
+    """A batch large enough that the reverse-step kernels run in several
+    rounds of workgroups (4 shots x 730 workgroups of 256 threads per
+    launch), with nonzero values in almost every workgroup's cells: a kernel
+    that read a value another workgroup of the same launch writes would
+    differ from the oracle here."""
+    print("discrete, several rounds of workgroups", flush=True)
+    O.set_threaded(True)
+    try:
+        base = RC.small_problem(nx=620, nz=280, npml=8, nt=10, fdorder=4, fsurf=1, seed=4)
+        rng = np.random.default_rng(21)
+        xr = list(np.linspace(base.ppad, base.nxt - base.ppad - 1, 32).astype(int))
+        rec = dict(x=xr * 2, z=[base.fpad + 1] * 32 + [base.nzt // 2] * 32, comp=[2] * 63 + [1])
+        shots, obs, refs = [], [], []
+        for i in range(4):
+            nt = 120 - 7 * i
+            xs = [100 + 11 * i, 300 - 13 * i, 500 + 5 * i]
+            zs = [base.nzt // 3 + i] * 3
+            sig = np.vstack([O.create_wavelet(nt, base.dt, 1.0 + 0.2 * k, 25.0, 0.0) for k in range(3)])
+            shots.append(dict(nt=nt, src=dict(x=xs, z=zs, comp=[2, 2, 1]), src_signal=sig, rec=rec))
+            pb = RC.small_problem(nx=620, nz=280, npml=8, nt=nt, fdorder=4, fsurf=1, seed=4)
+            pb.src = dict(x=xs, z=zs, comp=[2, 2, 1], shot=[0] * 3)
+            pb.src_signal = sig
+            pb.rec = dict(rec, shot=[0] * 64)
+            d = O.forward(pb)["rec"]
+            o = d * 0.5 + 1e-3 * rng.standard_normal(d.shape)
+            obs.append(o)
+            _, res, _ = O.l2_adjoint_sources(pb.dt, d.copy(), o)
+            refs.append(O.discrete_gradient(pb, res))
+    finally:
+        O.set_threaded(False)
+    plan, _ = _device_gradient(base, obs, "fp64", 16, shots=shots)
+    for i in range(len(shots)):
+        g = plan.coefficient_gradient(i)
+        for k in COEF:
+            assert (refs[i][k] != 0).mean() > 0.5, (i, k)
+            assert np.array_equal(g[k], refs[i][k]), (i, k)
+
+
 def test_discrete_mode_errors():
     """Explosive sources and a CPML profile with a != 0 at the inner edge are
     refused with a message; the reference-mode getters stay unavailable."""
