@@ -16,6 +16,9 @@
 
 namespace psv {
 
+// workgroup shape of the reverse-step kernels: 64 x 4 cells
+constexpr int kDiscreteBX = 64, kDiscreteBZ = 4;
+
 template <typename T>
 struct DiscreteParams {
     int nzt, nxt, pitch;
@@ -51,9 +54,27 @@ struct DiscreteParams {
     const size_t* sig_off;
     const int* shot_nt;
     int max_groups;
+    // receiver adjoint sources added by the receiver kernel at p.it: slots
+    // [inj_lo, inj_hi) (the others get theirs from the previous launch's
+    // stress pass)
+    int inj_lo, inj_hi;
+    // velocity pass: first add the velocity adjoints gathered from the stress
+    // weights of the previous reverse step (0 for a batch's first step)
+    int gather_prev;
+    // stress pass: also add the receiver adjoint sources of time index
+    // inj_next (the next reverse step; -1: none), through the receiver
+    // groups of each 64 x 4 workgroup (CSR over slot * nblk + block)
+    int inj_next, nblk;
+    const int* blk_off;
+    const int* blk_idx;
 };
 
-// One reverse time step of the active slots (p.nslots) at time index p.it.
+// One reverse time step of the active slots (p.nslots) at time index p.it:
+// the receiver kernel (slots [inj_lo, inj_hi)), the velocity pass (gather of
+// the previous step's stress weights, then the velocity weights) and the
+// stress pass (gather, transposed mirror, stress weights, and the next step's
+// receiver sources).  The velocity adjoints gathered from the last step's
+// stress weights are never added: nothing reads them.
 template <typename T>
 hipError_t launch_discrete_step(const DiscreteParams<T>& p, int order, hipStream_t s);
 

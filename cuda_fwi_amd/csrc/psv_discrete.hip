@@ -9,10 +9,17 @@
 // derivative terms (through the transposed CPML recursions), and a gather
 // pass that applies the transposed difference stencils to those weights.
 // Splitting at the weights keeps every adjoint update a gather (no
-// scatter, no atomics) and the order of every sum fixed.  A reverse step is
-// three launches after the receivers: velocity weights; stress gather,
-// transposed mirror and stress weights (the stress weights need the stress
-// adjoints of their own cell only); velocity gather.
+// scatter, no atomics) and the order of every sum fixed.  A weight pass
+// needs the adjoints of its own cell only, so each gather runs in the
+// launch of the next weight pass:
+//   velocity pass: velocity adjoints += gather of the previous reverse
+//     step's stress weights (after that step's receiver sources), then the
+//     velocity weights;
+//   stress pass: stress adjoints += gather of the velocity weights, the
+//     transposed mirror, the stress weights, and the next reverse step's
+//     receiver sources.
+// A reverse step is these two launches, plus a receiver launch for the
+// shots whose first reverse step it is.
 //
 // The per-cell expressions are fixed: the CPU restatement evaluates the
 // same ones in the same order.
@@ -24,7 +31,7 @@ namespace psv {
 
 namespace {
 
-constexpr int kDX = 64, kDZ = 4;
+constexpr int kDX = kDiscreteBX, kDZ = kDiscreteBZ;
 
 template <typename T>
 struct Cell {
@@ -43,27 +50,62 @@ __device__ __forceinline__ T cpml_t(T w, T* psib, T a, T b, T K)
     return wn;
 }
 
+// the receiver adjoint sources of one group of co-located receivers, time index it
 template <typename T>
-__global__ void d_inject_kernel(const DiscreteParams<T> p)
+__device__ __forceinline__ void add_receiver_group(const DiscreteParams<T>& p, int slot, int gid, int it)
 {
-    const int slot = blockIdx.y;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g0 = p.grp_off[slot], g1 = p.grp_off[slot + 1];
-    if (g0 + g >= g1) return;
-    const int* e = p.grp + 5 * (size_t)(g0 + g);
+    const int* e = p.grp + 5 * (size_t)gid;
     const int comp = e[2];
     if (comp != 1 && comp != 2) return;
     const int nt_s = p.shot_nt[slot];
     T* v = (comp == 1 ? p.vbx : p.vbz) + (size_t)slot * p.fst + (long)e[0] * p.pitch + e[1];
-    const T* sig = p.sig + p.sig_off[slot] + p.it;
+    const T* sig = p.sig + p.sig_off[slot] + it;
     T acc = *v;
     for (int q = 0; q < e[4]; q++) acc = acc + p.dt * sig[(size_t)p.grp_row[e[3] + q] * nt_s];
     *v = acc;
 }
 
+template <typename T>
+__global__ void d_inject_kernel(const DiscreteParams<T> p)
+{
+    const int slot = p.inj_lo + (int)blockIdx.y;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g0 = p.grp_off[slot], g1 = p.grp_off[slot + 1];
+    if (g0 + g >= g1) return;
+    add_receiver_group(p, slot, g0 + g, p.it);
+}
+
+// increments of the velocity adjoints at offset k: the transposed velocity
+// differences of the stress half-step's weights (interior cells: the
+// velocities elsewhere never change)
+template <typename T, int ORDER>
+__device__ __forceinline__ void stress_gather_inc(const DiscreteParams<T>& p, size_t k, T& ix_, T& iz_)
+{
+    const Cell<T> W0{p.ws0 + k, p.pitch}, W1{p.ws1 + k, p.pitch}, W2{p.ws2 + k, p.pitch},
+        W3{p.ws3 + k, p.pitch};
+    const T dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
+    if (ORDER == 2) {
+        const T xx = dxi * hc1 * (W0(0, 0) - W0(0, 1));
+        const T xz = dzi * hc1 * (W2(-1, 0) - W2(0, 0));
+        ix_ = xx + xz;
+        const T zx = dxi * hc1 * (W1(0, -1) - W1(0, 0));
+        const T zz = dzi * hc1 * (W3(0, 0) - W3(1, 0));
+        iz_ = zx + zz;
+    } else {
+        const T xx = dxi * (hc1 * (W0(0, 0) - W0(0, 1)) + hc2 * (W0(0, -1) - W0(0, 2)));
+        const T xz = dzi * (hc1 * (W2(-1, 0) - W2(0, 0)) + hc2 * (W2(-2, 0) - W2(1, 0)));
+        ix_ = xx + xz;
+        const T zx = dxi * (hc1 * (W1(0, -1) - W1(0, 0)) + hc2 * (W1(0, -2) - W1(0, 1)));
+        const T zz = dzi * (hc1 * (W3(0, 0) - W3(1, 0)) + hc2 * (W3(-1, 0) - W3(2, 0)));
+        iz_ = zx + zz;
+    }
+}
+
+// The velocity side of a reverse step: the velocity adjoints gathered from
+// the previous reverse step's stress weights (when p.gather_prev), then the
 // transposed velocity half-step: weights and the rho_xp / rho_zp gradients
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kDX* kDZ) void d_vel_weights_kernel(const DiscreteParams<T> p)
+__global__ __launch_bounds__(kDX* kDZ) void d_vel_pass_kernel(const DiscreteParams<T> p)
 {
     constexpr int H = ORDER / 2;
     const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
@@ -132,7 +174,15 @@ __global__ __launch_bounds__(kDX* kDZ) void d_vel_weights_kernel(const DiscreteP
         }
     }
     const size_t k = so + c;
-    const T bx = p.vbx[k], bz = p.vbz[k];
+    T bx = p.vbx[k], bz = p.vbz[k];
+    if (p.gather_prev) {
+        T gx, gz;
+        stress_gather_inc<T, ORDER>(p, k, gx, gz);
+        bx = bx + gx;
+        bz = bz + gz;
+        p.vbx[k] = bx;
+        p.vbz[k] = bz;
+    }
     p.g_rho_xp[k] = p.g_rho_xp[k] + bx * (dt * (sxx_x + szx_z));
     p.g_rho_zp[k] = p.g_rho_zp[k] + bz * (dt * (szx_x + szz_z));
     const T ax = dt * p.rho_xp[c] * bx;
@@ -203,7 +253,8 @@ __device__ __forceinline__ void vel_gather_inc(const DiscreteParams<T>& p, size_
 //      increments (the ghost rows' adjoints are zero before every gather and
 //      are never stored);
 //   3. the transposed stress half-step: weights and the lam / mu / mu_zx
-//      gradients.
+//      gradients;
+// and the receiver adjoint sources of the next reverse step.
 // Every value is the one the three steps give as separate passes over the
 // whole grid, bit for bit.
 template <typename T, int ORDER>
@@ -212,6 +263,14 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_pass_kernel(const DiscreteP
     constexpr int H = ORDER / 2;
     const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
     const int slot = blockIdx.z;
+    if (p.inj_next >= 0) {
+        // the next reverse step's receiver sources in this workgroup's cells
+        // (velocity adjoints: nothing else in this pass touches them)
+        const int tix = slot * p.nblk + (int)(blockIdx.y * gridDim.x + blockIdx.x);
+        const int q1 = p.blk_off[tix + 1];
+        for (int q = p.blk_off[tix] + (int)(threadIdx.y * kDX + threadIdx.x); q < q1; q += kDX * kDZ)
+            add_receiver_group(p, slot, p.blk_idx[q], p.inj_next);
+    }
     const int nzt = p.nzt, nxt = p.nxt, ppad = p.ppad;
     if (iz < H || iz >= nzt - H || ix < H || ix >= nxt - H) return;
     const size_t so = (size_t)slot * p.fst;
@@ -363,46 +422,16 @@ __global__ __launch_bounds__(kDX* kDZ) void d_stress_pass_kernel(const DiscreteP
     p.ws3[k] = w_vz_z;
 }
 
-// velocity adjoints += transposed velocity differences of the weights
-// (interior cells: the velocities elsewhere never change)
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kDX* kDZ) void d_stress_gather_kernel(const DiscreteParams<T> p)
-{
-    constexpr int H = ORDER / 2;
-    const int ix = blockIdx.x * kDX + threadIdx.x, iz = blockIdx.y * kDZ + threadIdx.y;
-    if (iz < H || iz >= p.nzt - H || ix < H || ix >= p.nxt - H) return;
-    const size_t k = (size_t)blockIdx.z * p.fst + (long)iz * p.pitch + ix;
-    const Cell<T> W0{p.ws0 + k, p.pitch}, W1{p.ws1 + k, p.pitch}, W2{p.ws2 + k, p.pitch},
-        W3{p.ws3 + k, p.pitch};
-    const T dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
-    if (ORDER == 2) {
-        const T xx = dxi * hc1 * (W0(0, 0) - W0(0, 1));
-        const T xz = dzi * hc1 * (W2(-1, 0) - W2(0, 0));
-        p.vbx[k] = p.vbx[k] + (xx + xz);
-        const T zx = dxi * hc1 * (W1(0, -1) - W1(0, 0));
-        const T zz = dzi * hc1 * (W3(0, 0) - W3(1, 0));
-        p.vbz[k] = p.vbz[k] + (zx + zz);
-    } else {
-        const T xx = dxi * (hc1 * (W0(0, 0) - W0(0, 1)) + hc2 * (W0(0, -1) - W0(0, 2)));
-        const T xz = dzi * (hc1 * (W2(-1, 0) - W2(0, 0)) + hc2 * (W2(-2, 0) - W2(1, 0)));
-        p.vbx[k] = p.vbx[k] + (xx + xz);
-        const T zx = dxi * (hc1 * (W1(0, -1) - W1(0, 0)) + hc2 * (W1(0, -2) - W1(0, 1)));
-        const T zz = dzi * (hc1 * (W3(0, 0) - W3(1, 0)) + hc2 * (W3(-1, 0) - W3(2, 0)));
-        p.vbz[k] = p.vbz[k] + (zx + zz);
-    }
-}
-
 template <typename T, int ORDER>
 hipError_t discrete_step(const DiscreteParams<T>& p, hipStream_t s)
 {
     const dim3 blk(kDX, kDZ);
     const dim3 grd((p.nxt + kDX - 1) / kDX, (p.nzt + kDZ - 1) / kDZ, p.nslots);
-    if (p.max_groups > 0)
-        hipLaunchKernelGGL((d_inject_kernel<T>), dim3((p.max_groups + 63) / 64, p.nslots), dim3(64), 0,
-                           s, p);
-    hipLaunchKernelGGL((d_vel_weights_kernel<T, ORDER>), grd, blk, 0, s, p);
+    if (p.max_groups > 0 && p.inj_hi > p.inj_lo)
+        hipLaunchKernelGGL((d_inject_kernel<T>), dim3((p.max_groups + 63) / 64, p.inj_hi - p.inj_lo),
+                           dim3(64), 0, s, p);
+    hipLaunchKernelGGL((d_vel_pass_kernel<T, ORDER>), grd, blk, 0, s, p);
     hipLaunchKernelGGL((d_stress_pass_kernel<T, ORDER>), grd, blk, 0, s, p);
-    hipLaunchKernelGGL((d_stress_gather_kernel<T, ORDER>), grd, blk, 0, s, p);
     return hipGetLastError();
 }
 

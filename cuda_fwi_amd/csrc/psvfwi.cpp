@@ -973,6 +973,32 @@ public:
         for (int sl = 0; sl < nslots_; sl++) maxg = std::max(maxg, go[(size_t)sl + 1] - go[(size_t)sl]);
         PSV_HIP(dgrp_off_.alloc(go.size()));
         PSV_HIP(hipMemcpy(dgrp_off_.p, go.data(), go.size() * sizeof(int), hipMemcpyHostToDevice));
+        // the receiver groups of each (slot, 64 x 4 workgroup of the reverse-step kernels)
+        {
+            const int bx = (g_.nxt + psv::kDiscreteBX - 1) / psv::kDiscreteBX;
+            const int bz = (g_.nzt + psv::kDiscreteBZ - 1) / psv::kDiscreteBZ;
+            dnblk_ = bx * bz;
+            std::vector<int> cnt((size_t)nslots_ * dnblk_ + 1, 0);
+            std::vector<std::pair<size_t, int>> items;
+            for (int sl = 0; sl < nslots_; sl++)
+                for (int g = go[(size_t)sl]; g < go[(size_t)sl + 1]; g++) {
+                    const int z = adj_tab_.grp_zx[2 * (size_t)g], x = adj_tab_.grp_zx[2 * (size_t)g + 1];
+                    const size_t tix = (size_t)sl * dnblk_ + (size_t)(z / psv::kDiscreteBZ) * bx +
+                                       (size_t)(x / psv::kDiscreteBX);
+                    items.emplace_back(tix, g);
+                }
+            std::stable_sort(items.begin(), items.end(),
+                             [](const std::pair<size_t, int>& a, const std::pair<size_t, int>& b) { return a.first < b.first; });
+            for (const auto& q : items) cnt[q.first + 1]++;
+            for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+            std::vector<int> idx;
+            for (const auto& q : items) idx.push_back(q.second);
+            PSV_HIP(dblk_off_.alloc(cnt.size()));
+            PSV_HIP(dblk_idx_.alloc(std::max<size_t>(idx.size(), 1)));
+            PSV_HIP(hipMemcpy(dblk_off_.p, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice));
+            if (!idx.empty())
+                PSV_HIP(hipMemcpy(dblk_idx_.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
 
         PSV_HIP(hipEventRecord(ev0_, stream_));
         launches_ = 0;
@@ -998,6 +1024,15 @@ public:
                 for (int it = t1 - 1; it >= t0; it--) {
                     psv::DiscreteParams<T> d = dparams(it, entry(it - t0), entry(it - t0 + 1));
                     d.max_groups = maxg;
+                    // receivers: the stress pass of step it + 1 added the sources
+                    // of the shots active there; the rest start at step it
+                    d.inj_lo = active(it + 1);
+                    d.inj_hi = d.nslots;
+                    d.gather_prev = it + 1 < ntg;
+                    d.inj_next = it > 0 ? it - 1 : -1;
+                    d.nblk = dnblk_;
+                    d.blk_off = plus(dblk_off_.p, (size_t)w0_ * dnblk_);
+                    d.blk_idx = dblk_idx_.p;
                     PSV_HIP(psv::launch_discrete_step<T>(d, g_.fdorder, stream_));
                     launches_++;
                 }
@@ -1914,7 +1949,8 @@ private:
     DBuf<T> neg_sig_;
     State<T> adj_;
     DBuf<T> dw_[8], dg_[5], seg_;  // dw_: velocity weights 0-3, stress weights 4-7
-    DBuf<int> dgrp_off_;
+    DBuf<int> dgrp_off_, dblk_off_, dblk_idx_;
+    int dnblk_ = 0;
     std::vector<double> h_pml_, h_mu_, h_mu_zx_, h_rho_zp_, h_rho_xp_;
     bool have_obs_ = false;
     double last_ms_ = 0;
