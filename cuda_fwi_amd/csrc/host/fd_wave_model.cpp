@@ -136,6 +136,7 @@ void clean_update(real_sim** mat, real_sim** mat_new, real_sim** grad, double st
 
 struct FDWaveModel::Impl {
     psvfwi_plan* plan = nullptr;
+    bool tuned = false;                               // the autotuner has run on this plan
     std::vector<std::vector<int>> src_idx, rec_idx;  // per shot: source / receiver entries
     ~Impl()
     {
@@ -226,6 +227,7 @@ static void build_plan(FDWaveModel& m, FDWaveModel::Impl* impl, bool with_window
     for (int i = 0; i < 7; i++) g.hc[i] = m.hc[i];
     if (impl->plan) psvfwi_plan_destroy(impl->plan);
     impl->plan = nullptr;
+    impl->tuned = false;
     check(psvfwi_plan_create(&g, m.dev.precision == "fp64" ? PSVFWI_FP64 : PSVFWI_FP32, &impl->plan), "plan_create");
     psvfwi_plan* p = impl->plan;
     if (m.pml.npml > 0) {
@@ -269,7 +271,7 @@ static void build_plan(FDWaveModel& m, FDWaveModel::Impl* impl, bool with_window
                                     (int)ri.size(), b[3].data(), b[4].data(), b[5].data()};
     }
     check(psvfwi_set_shots(p, (int)m.nshots, shots.data()), "set_shots");
-    if (m.dev.tile > 0) check(psvfwi_set_tile(p, m.dev.tile), "set_tile");
+    if (m.dev.tile != 0) check(psvfwi_set_tile(p, m.dev.tile), "set_tile");
 }
 
 static void upload_model(FDWaveModel& m, psvfwi_plan* p)
@@ -278,6 +280,16 @@ static void upload_model(FDWaveModel& m, psvfwi_plan* p)
     check(psvfwi_set_model(p, m.medium.lam[0], m.medium.mu[0], m.medium.mu_zx[0], m.medium.rho_zp[0],
                            m.medium.rho_xp[0]),
           "set_model");
+}
+
+// [mi355x] tile = 0: once per plan, after the first model upload, the
+// autotuner picks the step kernel, tile height, shot order and launch group on
+// the plan's own shots (results do not depend on the choice)
+static void tune_once(FDWaveModel& m, FDWaveModel::Impl* impl)
+{
+    if (m.dev.tile != 0 || impl->tuned) return;
+    check(psvfwi_autotune(impl->plan, std::min(100, (int)m.grid.nt), nullptr), "autotune");
+    impl->tuned = true;
 }
 
 static void download_records(FDWaveModel& m, FDWaveModel::Impl* impl)
@@ -330,6 +342,7 @@ void FDWaveModel::forward_simulate()
     build_plan(*this, impl_, false);
     psvfwi_plan* p = impl_->plan;
     upload_model(*this, p);
+    tune_once(*this, impl_);
     const bool snaps = dev.write_csv && grid.snap_interval > 0;
     check(psvfwi_set_snapshots(p, snaps ? grid.snap_interval : 0, 4), "set_snapshots");
     check(psvfwi_forward(p, PSVFWI_ENERGY), "forward");
@@ -415,6 +428,7 @@ real_sim FDWaveModel::fwi_step_length_estimation(real_sim est, real_sim* L2_test
         upd(medium.mu_old, medium.mu, fwi.grad_mu, est, 3);
         upd(medium.rho_old, medium.rho, fwi.grad_rho, rho_factor * est, 1);
         upload_model(*this, impl_->plan);
+        tune_once(*this, impl_);
         L2_test[itest] = batch_misfit(*this, impl_, 0);
         step_length[itest] = est;
         step_length[0] = 0.0;
@@ -509,6 +523,7 @@ void FDWaveModel::fwi_simulate()
     if (dev.observed == "modified") {
         medium.stagger_over_grid_modify(nzt, nxt);
         upload_model(*this, p);
+        tune_once(*this, impl_);
         check(psvfwi_forward(p, 0), "forward");
         download_records(*this, impl_);
         for (unsigned int i = 0; i < receiver.nseis; i++)
@@ -546,6 +561,7 @@ void FDWaveModel::fwi_simulate()
         std::cout << "Full Waveform Inversion iteration " << iter++ << std::endl;
         real_sim C_lam = 0, C_mu = 0, C_rho = 0;
         upload_model(*this, p);
+        tune_once(*this, impl_);
         medium.medium_average(C_lam, C_mu, C_rho, nzt, nxt);
 
         real_sim L2_test[4] = {0, 0, 0, 0};
