@@ -185,6 +185,7 @@ def run_c2(args):
     fwi_dt = args.fwi_dt
     window = w.model_window(dt=fwi_dt)
     plan = make_plan(w, args.precision, window, args.store, args.interval, args.tile)
+    plan.set_graph_steps(args.graph_steps)
     _log("setup done (observed data computed)")
     esize = 4 if args.precision == "fp32" else 8
 
@@ -260,7 +261,8 @@ def run_c2(args):
                    "fwi_time_decimation": fwi_dt,
                    "tile": [64, abs(plan.tile)],
                    "step_kernel": "fused (one launch per step)" if plan.tile < 0 else "split (stress, velocity)",
-                   "tile_choice": "autotuned" if args.tile == 0 else "fixed"},
+                   "tile_choice": "autotuned" if args.tile == 0 else "fixed",
+                   "graph_steps": args.graph_steps},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[kname],
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
