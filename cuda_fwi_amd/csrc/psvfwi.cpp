@@ -1712,7 +1712,7 @@ private:
     // Launches one step; on every kSampleEvery-th step of a timed loop the
     // two kernels are bracketed by events (kind 0 forward, 1 adjoint,
     // 2 checkpoint replay), read back by collect_samples().
-    static constexpr int kSampleEvery = 16;
+    static constexpr int kSampleEvery = 64;
     static constexpr int kMaxSamples = 512;
 
     // one step on st: the split kernels update st in place; the fused kernel

@@ -242,7 +242,7 @@ psvfwi_status psvfwi_chain_gradient(int nzt, int nxt, const double* mu, const do
  * number of step-kernel launches it made. */
 psvfwi_status psvfwi_last_timing(const psvfwi_plan* plan, double* ms, long* launches);
 
-/* Per-kernel durations sampled (HIP events on the plan's stream, every 16th
+/* Per-kernel durations sampled (HIP events on the plan's stream, every 64th
  * step) during the last forward / adjoint call: kind 0 forward steps, 1
  * adjoint steps, 2 checkpoint replay steps; mean microseconds of the stress
  * kernel and of the velocity kernel, and the number of samples.  (A
