@@ -1,0 +1,175 @@
+"""The multi-shot FWI loop of cuda_fwi_amd/fwi.py on the CPU, against a
+stand-in plan whose misfit is a known quadratic of the model: the reference's
+bracketing step search (fwi_step_length_estimation.cu:11-224), the parabola
+vertex, the update bounds of fwi_grad_update (fd_medium.cu:161-240), the
+edge-replicated padding, the muted rows and the shot groups.  The device
+path of the same loop runs in test_gpu_fwi.py."""
+import numpy as np
+import pytest
+
+from cuda_fwi_amd import fwi as F
+from cuda_fwi_amd import psvfwi as P
+
+
+class QuadraticPlan:
+    """Stands in for psvfwi.Plan: shot s has misfit
+    J_s = 0.5 c_s sum_window [((lam - lam*) / 1e9)^2 + ((mu - mu*) / 1e9)^2],
+    its illumination is 1 everywhere, and model_gradient(-1) returns the
+    exact dJ/d(lam, mu, rho) summed over the current shots (rho: 0).
+    `flip` negates the gradient (a search that can only fail)."""
+
+    def __init__(self, truth, win, flip=False):
+        self.truth = truth
+        self.win = win
+        self.flip = flip
+        self.mode = None
+        self.calls = {"forward": 0, "adjoint": 0}
+
+    def set_gradient_mode(self, mode, segment):
+        self.mode = (mode, segment)
+
+    def set_shots(self, shots):
+        self.shots = list(shots)
+
+    def set_observed(self, observed):
+        self.observed = list(observed)
+
+    def set_model(self, lam, mu, mu_zx, rho_zp, rho_xp):
+        assert mu_zx.shape == (lam.shape[0] - 1, lam.shape[1] - 1)
+        self.lam, self.mu = lam.copy(), mu.copy()
+
+    def _terms(self):
+        z1, z2, x1, x2 = self.win
+        dl = (self.lam - self.truth[0])[z1:z2, x1:x2] / 1e9
+        dm = (self.mu - self.truth[1])[z1:z2, x1:x2] / 1e9
+        return dl, dm
+
+    def forward(self, energy=True, gradient=False):
+        self.calls["forward"] += 1
+        dl, dm = self._terms()
+        self.J = np.array([0.5 * s["c"] * (np.sum(dl * dl) + np.sum(dm * dm)) for s in self.shots])
+
+    def misfit(self):
+        return self.J.copy()
+
+    def field(self, name, shot):
+        assert name == "We"
+        return np.ones_like(self.lam)
+
+    def adjoint(self, energy=False, gradient=True):
+        self.calls["adjoint"] += 1
+
+    def model_gradient(self, shot=-1):
+        z1, z2, x1, x2 = self.win
+        dl, dm = self._terms()
+        c = sum(s["c"] for s in self.shots) * (-1.0 if self.flip else 1.0)
+        g = [np.zeros_like(self.lam) for _ in range(3)]
+        g[0][z1:z2, x1:x2] = c * dl / 1e9
+        g[1][z1:z2, x1:x2] = c * dm / 1e9
+        return g
+
+
+def _case(nz=14, nx=18, pad=3, seed=1):
+    rng = np.random.default_rng(seed)
+    shape = (nz + 2 * pad, nx + 2 * pad)
+    win = (pad, pad + nz, pad, pad + nx)
+    truth = [np.full(shape, 9.0e9), np.full(shape, 4.0e9), np.full(shape, 2200.0)]
+    truth[0][win[0]:win[1], win[2]:win[3]] += 1.0e9 * rng.random((nz, nx))
+    truth[1][win[0]:win[1], win[2]:win[3]] += 0.5e9 * rng.random((nz, nx))
+    start = [np.full(shape, 9.5e9), np.full(shape, 4.2e9), np.full(shape, 2200.0)]
+    shots = [dict(c=1.0 + 0.25 * i) for i in range(5)]
+    observed = [np.zeros((2, 3)) for _ in shots]
+    return truth, start, win, shots, observed
+
+
+def _inversion(plan, start, win, shots, observed, **kw):
+    return F.Inversion(plan, shots, observed, start, win, segment=40, step=0.01, **kw)
+
+
+def test_misfit_falls_and_model_approaches_the_truth():
+    truth, start, win, shots, observed = _case()
+    plan = QuadraticPlan(truth, win)
+    inv = _inversion(plan, start, win, shots, observed)
+    assert plan.mode == ("discrete", 40)
+    z1, z2, x1, x2 = win
+    err0 = np.abs(inv.m["lam"] - truth[0])[z1:z2, x1:x2].max()
+    h = inv.run(6)
+    assert all(b < a for a, b in zip(h.misfit, h.misfit[1:]))
+    assert all(s > 0 for s in h.step)
+    assert all(2 <= t <= 16 for t in h.trials)
+    assert np.abs(inv.m["lam"] - truth[0])[z1:z2, x1:x2].max() < err0
+    # rho has a zero gradient: untouched
+    assert np.array_equal(inv.m["rho"], start[2])
+    # the padding is edge replication of the window
+    for k in ("lam", "mu"):
+        m = inv.m[k]
+        assert np.array_equal(m, F._repad(m, win))
+    # one gradient (forward + adjoint) and the trial forwards per iteration
+    assert plan.calls["adjoint"] == 6
+    assert plan.calls["forward"] == 6 + sum(h.trials)
+
+
+def test_parabola_vertex_on_a_quadratic():
+    """On an exactly quadratic misfit the parabola through the bracketing
+    points has its vertex at the minimum along the search direction: the
+    accepted step beats every step of a scan around it."""
+    truth, start, win, shots, observed = _case()
+    inv = _inversion(QuadraticPlan(truth, win), start, win, shots, observed)
+    _, ill, g = inv._gradient()
+    z1, z2, x1, x2 = win
+    w = ill + 0.005 * ill[z1:z2, x1:x2].max()
+    d = {k: -gk / w for k, gk in zip(("lam", "mu", "rho"), g)}
+    L2, step, _ = inv.iterate()
+    ref = _inversion(QuadraticPlan(truth, win), start, win, shots, observed)  # still at the start model
+    ref.plan.set_shots(shots)
+
+    def J(s):
+        t = ref._trial(d, s)
+        ref.plan.set_model(t["lam"], t["mu"], *P.average_parameters(t["mu"], t["rho"]))
+        ref.plan.forward()
+        return float(np.sum(ref.plan.misfit()))
+
+    j_opt = J(step)
+    assert j_opt < L2
+    assert all(j_opt <= J(s) * (1 + 1e-9) for s in np.linspace(0.5, 1.5, 21) * step)
+
+
+def test_shot_groups_equal_one_batch():
+    truth, start, win, shots, observed = _case()
+    a = _inversion(QuadraticPlan(truth, win), start, win, shots, observed)
+    b = _inversion(QuadraticPlan(truth, win), start, win, shots, observed, group=2)
+    assert [len(g) for g in b.groups] == [2, 2, 1]
+    ha, hb = a.run(3), b.run(3)
+    assert np.allclose(ha.misfit, hb.misfit, rtol=1e-12, atol=0)
+    assert np.allclose(ha.step, hb.step, rtol=1e-9, atol=0)
+    for k in a.m:
+        assert np.allclose(a.m[k], b.m[k], rtol=1e-9, atol=0)
+
+
+def test_muted_rows_and_bounds():
+    truth, start, win, shots, observed = _case()
+    z1, z2, x1, x2 = win
+    inv = _inversion(QuadraticPlan(truth, win), start, win, shots, observed, mute=4)
+    inv.run(2)
+    for k in ("lam", "mu"):
+        assert np.array_equal(inv.m[k][z1:z1 + 4, x1:x2], start[{"lam": 0, "mu": 1}[k]][z1:z1 + 4, x1:x2])
+    # _update clips to the fwi_grad_update bounds and scales by max(m) / max|d|
+    m = np.full((6, 6), 10.0e9)
+    d = np.zeros((6, 6))
+    d[2, 3] = -5.0
+    d[3, 3] = 2.5
+    out = F._update(m, d, 0.1, (1, 5, 1, 5), F.BOUNDS["lam"])
+    assert out[2, 3] == pytest.approx(10.0e9 - 0.1 * 10.0e9)
+    assert out[3, 3] == pytest.approx(10.0e9 + 0.05 * 10.0e9)
+    big = F._update(m, d, 10.0, (1, 5, 1, 5), F.BOUNDS["lam"])
+    assert big[2, 3] == F.BOUNDS["lam"][0] and big[3, 3] == F.BOUNDS["lam"][1]
+    assert np.array_equal(F._update(m, np.zeros_like(m), 0.1, (1, 5, 1, 5), F.BOUNDS["lam"]), m)
+
+
+def test_search_that_cannot_descend_raises():
+    """A direction along which the misfit only grows: the search halves the
+    step until its try count runs out and raises, as the reference exits."""
+    truth, start, win, shots, observed = _case()
+    inv = _inversion(QuadraticPlan(truth, win, flip=True), start, win, shots, observed)
+    with pytest.raises(F.StepSearchFailed):
+        inv.iterate()
