@@ -409,7 +409,7 @@ def run_c5(args):
     upd = 2.0 * cells * nt * nshots * args.iters
     return {
         "metric": METRIC, "value": round(upd / T / 1e9, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
-        "steps": args.iters, "warmup": 0, "ms_per_step": round(T / args.iters * 1e3, 1),
+        "steps": args.iters, "warmup": 0, "ms_per_step": round(T / max(args.iters, 1) * 1e3, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic: 10 layers, two faults and a vp 4500 irregular body (seed 7); start model = "

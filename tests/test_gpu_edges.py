@@ -182,3 +182,15 @@ def test_empty_window_gives_zero_store_and_gradient():
     plan.misfit([obs])
     plan.adjoint(gradient=True)
     assert not any(np.any(g) for g in plan.gradient(0))
+
+
+def test_tile_heights_refused():
+    """Tile heights outside the instantiated kernels are refused with a
+    message; the 64 x 32 split tile is FP32 only."""
+    from cuda_fwi_amd import psvfwi as P
+    print("tile heights refused", flush=True)
+    pb = _grid_problem(33, 65, 4, 8, 0, nt=10, src_at=(16, 40))
+    for prec, bad in (("fp32", 12), ("fp32", -32), ("fp32", -4), ("fp32", -24), ("fp64", 32)):
+        plan = plan_of(pb, prec)
+        with pytest.raises(P.PsvfwiError, match="tile height"):
+            plan.set_tile(bad)
