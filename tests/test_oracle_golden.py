@@ -244,13 +244,40 @@ def test_adjoint_propagation_is_time_reversed_forward(fsurf, fdorder):
         assert np.array_equal(adj[k], fw[k]), k
 
 
+def _compliance_pairings(L, M, fwd, adj):
+    """Per cell, -adj^T (dS/dlam) fwd and the three parts of -adj^T (dS/dmu) fwd along
+    the eigenmodes of S (pressure xx = zz, deviatoric xx = -zz, shear), where S = C^-1 is
+    the compliance of 2-D plane-strain isotropic elasticity, C = [[lam+2mu, lam, 0],
+    [lam, lam+2mu, 0], [0, 0, mu]] on Voigt vectors (sxx, szz, sxz).  S comes from
+    numpy's inverse and its derivatives from central differences (step 1e-4 relative),
+    so nothing here restates the reference's correlation formulas.  fwd, adj: (..., 3)."""
+    def S(lam, mu):
+        c = np.zeros(lam.shape + (3, 3))
+        c[..., 0, 0] = c[..., 1, 1] = lam + 2 * mu
+        c[..., 0, 1] = c[..., 1, 0] = lam
+        c[..., 2, 2] = mu
+        return np.linalg.inv(c)
+    hl, hm = (1e-4 * L)[..., None, None], (1e-4 * M)[..., None, None]
+    dSl = (S(L + hl[..., 0, 0], M) - S(L - hl[..., 0, 0], M)) / (2 * hl)
+    dSm = (S(L, M + hm[..., 0, 0]) - S(L, M - hm[..., 0, 0])) / (2 * hm)
+    form = lambda D, a, b: -np.einsum("...i,...ij,...j->...", a, D, b)
+    parts = []
+    for e in (np.array([1.0, 1.0, 0.0]) / np.sqrt(2), np.array([1.0, -1.0, 0.0]) / np.sqrt(2),
+              np.array([0.0, 0.0, 1.0])):
+        parts.append(form(dSm, (adj @ e)[..., None] * e, (fwd @ e)[..., None] * e))
+    return form(dSl, adj, fwd), parts
+
+
 def test_gradient_brute_force_tiny():
     """Re-evaluates the reference correlation (adjoint_kernel_PSV_GPU.cu:818-863)
     from independently produced wavefields: the adjoint state before step it is
     the forward state after nt-1-it steps driven by the reversed residual, the
     adjoint displacement is the running dt-weighted sum of those velocities.
     Checks the decimated window, the ordering (displacement updated before the
-    correlation), the v/dt store and every term's sign."""
+    correlation), the v/dt store and every term's sign.  The lambda and mu terms
+    come from the derivative of the plane-strain compliance (_compliance_pairings),
+    not from the reference's formulas: lambda's is -adj (dS/dlam) fwd; mu's is the
+    same derivative with its deviatoric part taken 4 times, the reference's s2."""
     nt = 23
     pb = RC.small_problem(nx=10, nz=9, npml=3, nt=nt, fsurf=1, nsrc=1, nrec=3, dt=6e-4, fc=60.0)
     pb.fwi = dict(dt=3, dx=2, dz=2, x1=pb.ppad - 1, x2=pb.nxt - 2, z1=pb.fpad, z2=pb.nzt - 3)
@@ -287,6 +314,7 @@ def test_gradient_brute_force_tiny():
     ux = np.zeros((pb.nzt, pb.nxt))
     uz = np.zeros((pb.nzt, pb.nxt))
     gl = np.zeros((nfz, nfx)); gm = np.zeros((nfz, nfx)); gr = np.zeros((nfz, nfx))
+    gmx = np.zeros((nfz, nfx))
     lam, mu = pb.lam, pb.mu
     w = pb.fwi
     zs = np.arange(w["z1"], w["z2"], w["dz"])
@@ -300,17 +328,22 @@ def test_gradient_brute_force_tiny():
             sel = np.ix_(zs, xs)
             fsxx, fszz, fszx = F["sxx"][sel], F["szz"][sel], F["szx"][sel]
             fvx, fvz = F["vx"][sel] / dt, F["vz"][sel] / dt
-            L, M = lam[sel], mu[sel]
-            s1 = (fsxx + fszz) * (a["sxx"][sel] + a["szz"][sel]) * 0.25 / (L + M) ** 2
-            s2 = (fsxx - fszz) * (a["sxx"][sel] - a["szz"][sel]) / M ** 2
-            s3 = fszx * a["szx"][sel] / M ** 2
-            s4 = ux[sel] * fvx + uz[sel] * fvz
-            gl[:len(zs), :len(xs)] += w["dt"] * dt * s1
-            gm[:len(zs), :len(xs)] += w["dt"] * dt * (s1 + s2 + s3)
+            fwd = np.stack([fsxx, fszz, fszx], -1)
+            adj_s = np.stack([a["sxx"][sel], a["szz"][sel], a["szx"][sel]], -1)
+            dlam, (dmu_p, dmu_q, dmu_s) = _compliance_pairings(lam[sel], mu[sel], fwd, adj_s)
+            # kinetic pairing: adjoint displacement against the stored forward v / dt
+            s4 = np.einsum("i...,i...->...", np.stack([ux[sel], uz[sel]]), np.stack([fvx, fvz]))
+            gl[:len(zs), :len(xs)] += w["dt"] * dt * dlam
+            gm[:len(zs), :len(xs)] += w["dt"] * dt * (dmu_p + 4.0 * dmu_q + dmu_s)
+            gmx[:len(zs), :len(xs)] += w["dt"] * dt * (dmu_p + dmu_q + dmu_s)
             gr[:len(zs), :len(xs)] -= w["dt"] * dt * s4
     for ours, ref in ((adj["grad_lam"], gl), (adj["grad_mu"], gm), (adj["grad_rho"], gr)):
         assert np.abs(ref).max() > 0
-        assert np.allclose(ours, ref, rtol=1e-10, atol=1e-14 * np.abs(ref).max())
+        assert np.allclose(ours, ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
+    # the reference's mu kernel is not the compliance derivative: its deviatoric term
+    # (adjoint_kernel_PSV_GPU.cu:846-847) has no factor 1/4, so it is 4x the derivative's
+    # (replicated); the exact derivative differs visibly
+    assert np.abs(adj["grad_mu"] - gmx).max() > 1e-3 * np.abs(gmx).max()
     # the last decimated row/column is never written (strict < in the window loops)
     assert nfz > len(zs) or nfx > len(xs)
     assert np.all(adj["grad_lam"][len(zs):, :] == 0) and np.all(adj["grad_lam"][:, len(xs):] == 0)
