@@ -13,7 +13,9 @@
 //     untouched
 //   * the wavefields, We and the FWI store start from zero (the reference's
 //     kernels reset them, CUDA_week2/forward_kernel_PSV_GPU.cu:760-792)
-//   * adjoint gradients are added to grad_lam / grad_mu / grad_rho
+//   * the adjoint writes this shot's gradient into grad_lam / grad_mu / grad_rho,
+//     overwriting what they held (the reference zeroes them on entry,
+//     CUDA_week2/adjoint_kernel_PSV_GPU.cu:799-806)
 //   * forward snapshots (when !fwinv): szz every snap_interval steps into
 //     <dir>/vz_snap<k>.csv, k = 0, 1, ... (the reference's file name and
 //     ", "-separated rows, CUDA_week2/forward_kernel_PSV_GPU.cu:1064-1080)
@@ -285,10 +287,12 @@ void adjoint_kernel_PSV(int ishot, int nt, int nzt, int nxt, int fpad, int ppad,
         const size_t n = nfz * nfx;
         std::vector<double> gl(n), gm(n), gr(n);
         check(psvfwi_get_gradient(c.plan, 0, gl.data(), gm.data(), gr.data()), "get_gradient");
+        // one shot's gradient overwrites the arrays, as the reference sets them to zero on
+        // entry (CUDA_week2/adjoint_kernel_PSV_GPU.cu:799-806)
         for (size_t i = 0; i < n; i++) {
-            grad_lam[0][i] += gl[i];
-            grad_mu[0][i] += gm[i];
-            grad_rho[0][i] += gr[i];
+            grad_lam[0][i] = gl[i];
+            grad_mu[0][i] = gm[i];
+            grad_rho[0][i] = gr[i];
         }
     }
 }

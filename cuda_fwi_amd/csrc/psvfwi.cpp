@@ -251,6 +251,7 @@ public:
     {
         if (!w) {
             win_on_ = false;
+            drop_results();
             return PSVFWI_OK;
         }
         if (w->dt < 1 || w->dx < 1 || w->dz < 1) return fail(PSVFWI_E_ARG, "set_window: strides must be >= 1");
@@ -260,7 +261,7 @@ public:
         nfz_ = 1 + (w->z2 - w->z1) / w->dz;
         nfx_ = 1 + (w->x2 - w->x1) / w->dx;
         win_on_ = true;
-        fwd_grad_done_ = false;
+        drop_results();
         return PSVFWI_OK;
     }
 
@@ -272,7 +273,7 @@ public:
             return fail(PSVFWI_E_ARG, "set_store_mode: checkpoint interval must be >= 1");
         store_mode_ = mode;
         ck_k_ = interval;
-        fwd_grad_done_ = false;
+        drop_results();
         return PSVFWI_OK;
     }
 
@@ -284,7 +285,7 @@ public:
             return fail(PSVFWI_E_ARG, "set_gradient_mode: segment length must be >= 1");
         grad_mode_ = mode;
         if (mode == PSVFWI_GRADIENT_DISCRETE) disc_k_ = interval;
-        fwd_grad_done_ = false;
+        drop_results();
         return PSVFWI_OK;
     }
 
@@ -454,8 +455,8 @@ public:
             }
         }
         if (best) *best = best_tz;
-        ran_ = false;
-        fwd_grad_done_ = false;
+        // the candidates ran in the plan's own state, store and gradient buffers
+        drop_results();
         return PSVFWI_OK;
     }
 
@@ -527,7 +528,8 @@ public:
         nt_slot_.resize((size_t)n);
         for (int s = 0; s < n; s++) nt_slot_[(size_t)s] = shots_[(size_t)shot_of_slot_[(size_t)s]].nt;
         PSV_TRY(upload_tables());
-        fwd_done_ = fwd_grad_done_ = have_obs_ = false;
+        drop_results();
+        have_obs_ = false;
         adj_ready_.assign((size_t)n, 0);
         return PSVFWI_OK;
     }
@@ -1260,6 +1262,8 @@ public:
         const int s = slot_of_shot_[(size_t)shot];
         const int nft = 1 + (shots_[(size_t)shot].nt - 1) / w_.dt;
         const size_t plane = (size_t)nfz_ * nfx_;
+        if (sst_ < (size_t)nft * 5 * plane || store_.n < (size_t)nslots_ * sst_)
+            return fail(PSVFWI_E_STATE, "get_store: the store does not match the current window and shots");
         std::vector<double> h(sst_);
         PSV_TRY(download(store_.p + (size_t)s * sst_, sst_, h.data()));
         // the window loops run over [z1, z2) x [x1, x2): when an extent is a
@@ -1284,6 +1288,8 @@ public:
         if (!adj_grad_done_) return fail(PSVFWI_E_STATE, "get_gradient: run adjoint with PSVFWI_GRADIENT first");
         if (shot < -1 || shot >= nslots_ || !gl || !gm || !gr) return fail(PSVFWI_E_ARG, "get_gradient: bad shot");
         const size_t plane = (size_t)nfz_ * nfx_;
+        if (gst_ < 3 * plane || grad_.n < (size_t)nslots_ * gst_)
+            return fail(PSVFWI_E_STATE, "get_gradient: the gradient does not match the current window and shots");
         std::vector<double> h(3 * plane);
         if (shot >= 0) {
             const int s = slot_of_shot_[(size_t)shot];
@@ -1628,6 +1634,15 @@ private:
             PSV_HIP(psv::launch_fill<T>(st.mv[m].p, st.mv[m].n, (T)0, stream_));
         }
         return PSVFWI_OK;
+    }
+
+    // Forgets every result held: a change of window, shots, store or gradient mode, or the
+    // autotuner's runs in the plan's buffers, leave them unreadable (the getters refuse).
+    void drop_results()
+    {
+        ran_ = false;
+        fwd_done_ = fwd_grad_done_ = store_loaded_ = adj_grad_done_ = false;
+        fwd_disc_ = disc_done_ = fwd_rb_ = false;
     }
 
     psvfwi_status alloc_store(int ntf)

@@ -94,6 +94,15 @@ def _dptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _f64(a, shape, what):
+    """a as a C-contiguous FP64 array of exactly this shape: the library copies
+    shape-product elements from the pointer, so a mismatch is refused here."""
+    a = np.ascontiguousarray(a, np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError(f"{what}: expected shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
 def _iptr(a):
     return a.ctypes.data_as(_ip)
 
@@ -125,11 +134,13 @@ class Plan:
             pass
 
     def set_pml(self, profiles):
-        prof = np.ascontiguousarray(profiles, np.float64)
+        prof = _f64(profiles, (6, 2 * (self.grid["npml"] + 1)), "set_pml")
         _check(lib().psvfwi_set_pml(self._h, _dptr(prof)))
 
     def set_model(self, lam, mu, mu_zx, rho_zp, rho_xp):
-        arrs = [np.ascontiguousarray(a, np.float64) for a in (lam, mu, mu_zx, rho_zp, rho_xp)]
+        nzt, nxt = self.grid["nzt"], self.grid["nxt"]
+        arrs = [_f64(a, (nzt, nxt) if k < 2 else (nzt - 1, nxt - 1), "set_model")
+                for k, a in enumerate((lam, mu, mu_zx, rho_zp, rho_xp))]
         _check(lib().psvfwi_set_model(self._h, *[_dptr(a) for a in arrs]))
 
     def set_window(self, win):
@@ -208,9 +219,10 @@ class Plan:
     def forward(self, energy=True, gradient=False):
         _check(lib().psvfwi_forward(self._h, (ENERGY if energy else 0) | (GRADIENT if gradient else 0)))
 
-    @staticmethod
-    def _concat(observed):
-        parts = [np.ascontiguousarray(o, np.float64).ravel() for o in observed]
+    def _concat(self, observed):
+        if len(observed) != len(self._shots):
+            raise ValueError(f"observed: expected {len(self._shots)} shots, got {len(observed)}")
+        parts = [_f64(o, (s["nrec"], s["nt"]), "observed").ravel() for o, s in zip(observed, self._shots)]
         return np.concatenate(parts) if parts and sum(p.size for p in parts) else np.zeros(1)
 
     def set_observed(self, observed):
@@ -229,11 +241,12 @@ class Plan:
         return out
 
     def set_adjoint_sources(self, shot, signal):
-        sig = np.ascontiguousarray(signal, np.float64)
+        s = self._shots[shot]
+        sig = _f64(signal, (s["nrec"], s["nt"]), "set_adjoint_sources")
         _check(lib().psvfwi_set_adjoint_sources(self._h, int(shot), _dptr(sig)))
 
     def set_store(self, shot, store):
-        st = np.ascontiguousarray(store, np.float64)
+        st = _f64(store, (5,) + self._fwi_dims(shot), "set_store")
         _check(lib().psvfwi_set_store(self._h, int(shot), _dptr(st)))
 
     def adjoint(self, energy=True, gradient=True):

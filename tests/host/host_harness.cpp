@@ -4,11 +4,14 @@
 //       ParseError, then per key: Get | GetInteger(-7) | GetReal(-7.5) | GetBoolean(default -1)
 //   host_harness config <file.ini> <out.txt>
 //       the configuration FDWaveModel parses, as "key value..." lines
-//   host_harness dropin <dir>
-//       forward_kernel_PSV + adjoint_kernel_PSV on the problem in <dir>/in.txt
-//       and <dir>/*.bin (raw float64), outputs to <dir>/out_*.bin; with
-//       snap_interval > 0 a second forward (fwinv = false) writes
+//   host_harness dropin <dir> [gpu]
+//       forward_kernel_PSV + adjoint_kernel_PSV (with "gpu": the _GPU entry points,
+//       null device pointers) on the problem in <dir>/in.txt and <dir>/*.bin (raw
+//       float64), outputs to <dir>/out_*.bin; the gradient arrays hold 7.0 on entry;
+//       with snap_interval > 0 a second forward (fwinv = false) writes
 //       <dir>/snap/vz_snap<k>.csv
+//   host_harness opt <L0> <L1> <L2> <s0> <s1> <s2> <out.txt>
+//       calc_opt_step of the three trial misfits and steps
 #include <cstdio>
 #include <filesystem>
 #include <fstream>
@@ -111,7 +114,20 @@ int run_config(char** argv)
     return 0;
 }
 
-int run_dropin(const std::string& dir)
+int run_opt(char** argv)
+{
+    real_sim L2[3], st[3];
+    for (int i = 0; i < 3; i++) {
+        L2[i] = std::stod(argv[2 + i]);
+        st[i] = std::stod(argv[5 + i]);
+    }
+    std::ofstream o(argv[8]);
+    o.precision(17);
+    o << calc_opt_step(L2, st) << "\n";
+    return 0;
+}
+
+int run_dropin(const std::string& dir, bool gpu)
 {
     std::ifstream in(dir + "/in.txt");
     int ishot, nt, nzt, nxt, fpad, ppad, snap, fsurf, fdorder, npml, nsrc, nrec, fwinv, fdt, fdx, fdz, x1, x2, z1, z2, fp64;
@@ -138,7 +154,7 @@ int run_dropin(const std::string& dir)
     const int nft = 1 + (nt - 1) / fdt, nfz = 1 + (z2 - z1) / fdz, nfx = 1 + (x2 - x1) / fdx;
     const size_t vol = (size_t)nft * nfz * nfx;
     std::vector<double> fld(6 * n), mem(4 * (size_t)nzt * tp + 4 * (size_t)tp * nxt), store(5 * vol),
-        grad((size_t)nfz * nfx, 0.0);
+        grad((size_t)nfz * nfx, 7.0);  // overwritten by the adjoint
     std::vector<std::vector<double*>> F;
     for (int k = 0; k < 6; k++) {
         std::vector<double> tmp;
@@ -173,22 +189,46 @@ int run_dropin(const std::string& dir)
     for (int k = 0; k < 6; k++) pr[k] = prof.data() + (size_t)k * tp;
 
     fd_kernel_set_snapshot_dir("");
-    forward_kernel_PSV(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
-                       F[1].data(), F[2].data(), F[3].data(), F[4].data(), F[5].data(), L.data(), U.data(), MZ.data(),
-                       RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], M[0].data(), M[4].data(),
-                       M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(), M[7].data(), nsrc, sx, sz, sc,
-                       SS.data(), ss, nrec, rx, rz, rc, RS.data(), rs, fwinv != 0, fdt, fdx, fdz, x1, x2, z1, z2,
-                       st[0], st[1], st[2], st[3], st[4]);
+    if (gpu) {
+        // the _GPU variants take the device buffers the reference's caller cudaMallocs;
+        // here the plan owns device memory, so they are passed as null
+        double* d = nullptr;
+        forward_kernel_PSV_GPU(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
+                               F[1].data(), F[2].data(), F[3].data(), F[4].data(), L.data(), U.data(), MZ.data(),
+                               RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], M[0].data(),
+                               M[4].data(), M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(),
+                               M[7].data(), nsrc, sx, sz, sc, SS.data(), ss, nrec, rx, rz, rc, RS.data(), rs,
+                               fwinv != 0, fdt, fdx, fdz, x1, x2, z1, z2, st[0], st[1], st[2], st[3], st[4], d, d,
+                               d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d);
+    } else {
+        forward_kernel_PSV(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
+                           F[1].data(), F[2].data(), F[3].data(), F[4].data(), F[5].data(), L.data(), U.data(),
+                           MZ.data(), RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5],
+                           M[0].data(), M[4].data(), M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(),
+                           M[7].data(), nsrc, sx, sz, sc, SS.data(), ss, nrec, rx, rz, rc, RS.data(), rs,
+                           fwinv != 0, fdt, fdx, fdz, x1, x2, z1, z2, st[0], st[1], st[2], st[3], st[4]);
+    }
     save(dir + "/out_rec.bin", rsig.data(), rsig.size());
     save(dir + "/out_fwd_fields.bin", fld.data(), fld.size());
     save(dir + "/out_fwd_mem.bin", mem.data(), mem.size());
     save(dir + "/out_store.bin", store.data(), store.size());
-    adjoint_kernel_PSV(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
-                       F[1].data(), F[2].data(), F[3].data(), F[4].data(), F[5].data(), L.data(), U.data(), MZ.data(),
-                       RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], M[0].data(), M[4].data(),
-                       M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(), M[7].data(), nrec, rx, rz, rc,
-                       AS.data(), rs, fwinv != 0, fdt, fdx, fdz, x1, x2, z1, z2, st[0], st[1], st[2], st[3], st[4],
-                       GL.data(), GM.data(), GR.data());
+    if (gpu) {
+        double* d = nullptr;
+        adjoint_kernel_PSV_GPU(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
+                               F[1].data(), F[2].data(), F[3].data(), F[4].data(), L.data(), U.data(), MZ.data(),
+                               RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5], M[0].data(),
+                               M[4].data(), M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(),
+                               M[7].data(), nrec, rx, rz, rc, AS.data(), rs, fwinv != 0, fdt, fdx, fdz, x1, x2, z1,
+                               z2, st[0], st[1], st[2], st[3], st[4], GL.data(), GM.data(), GR.data(), d, d, d, d, d,
+                               d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d, d);
+    } else {
+        adjoint_kernel_PSV(ishot, nt, nzt, nxt, fpad, ppad, dt, dx, dz, snap, fsurf, hc, fdorder, F[0].data(),
+                           F[1].data(), F[2].data(), F[3].data(), F[4].data(), F[5].data(), L.data(), U.data(),
+                           MZ.data(), RZ.data(), RX.data(), npml, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5],
+                           M[0].data(), M[4].data(), M[1].data(), M[5].data(), M[2].data(), M[3].data(), M[6].data(),
+                           M[7].data(), nrec, rx, rz, rc, AS.data(), rs, fwinv != 0, fdt, fdx, fdz, x1, x2, z1, z2,
+                           st[0], st[1], st[2], st[3], st[4], GL.data(), GM.data(), GR.data());
+    }
     save(dir + "/out_adj_fields.bin", fld.data(), fld.size());
     std::vector<double> g(grad);
     g.insert(g.end(), g2.begin(), g2.end());
@@ -215,11 +255,12 @@ int main(int argc, char** argv)
         const std::string mode = argc > 1 ? argv[1] : "";
         if (mode == "ini" && argc >= 4) return run_ini(argc, argv);
         if (mode == "config" && argc >= 4) return run_config(argv);
-        if (mode == "dropin" && argc >= 3) return run_dropin(argv[2]);
+        if (mode == "dropin" && argc >= 3) return run_dropin(argv[2], argc >= 4 && std::string(argv[3]) == "gpu");
+        if (mode == "opt" && argc >= 9) return run_opt(argv);
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
     }
-    std::cerr << "usage: host_harness ini|config|dropin ..." << std::endl;
+    std::cerr << "usage: host_harness ini|config|dropin|opt ..." << std::endl;
     return 2;
 }

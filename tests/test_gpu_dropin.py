@@ -53,11 +53,19 @@ def _write_case(d, pb, ishot, src_shot, rec_shot, adj, snap, fp64=1):
         np.ascontiguousarray(a, np.float64).tofile(d / f"{name}.bin")
 
 
+@pytest.mark.parametrize("entry", ["cpu_signature", "gpu_signature"])
 @pytest.mark.parametrize("fdorder", [2, 4])
-def test_dropin_kernels_fp64_bit_exact(harness, tmp_path, fdorder):
-    """forward_kernel_PSV then adjoint_kernel_PSV for shot 1 of a two-shot
-    geometry: traces, wavefields, CPML memory, FWI store, gradients."""
-    print(f"drop-in kernels order {fdorder}", flush=True)
+def test_dropin_kernels_fp64_bit_exact(harness, tmp_path, fdorder, entry):
+    """forward_kernel_PSV then adjoint_kernel_PSV (or the _GPU entry points of
+    CUDA_week2/fd_kernel.cuh:75-173, whose device pointers the plan replaces) for
+    shot 1 of a two-shot geometry: traces, wavefields, CPML memory, FWI store,
+    gradients.  The gradient arrays hold 7.0 on entry and must come back holding
+    this shot's gradient alone (the reference zeroes them,
+    CUDA_week2/adjoint_kernel_PSV_GPU.cu:799-806)."""
+    print(f"drop-in kernels order {fdorder}, {entry}", flush=True)
+    gpu = entry == "gpu_signature"
+    # the _GPU signatures have no We argument (fd_kernel.cuh:75-120)
+    fields = ("vx", "vz", "sxx", "szx", "szz") if gpu else ("vx", "vz", "sxx", "szx", "szz", "We")
     pb = RC.small_problem(nx=64, nz=40, npml=8, nt=240, fdorder=fdorder, fsurf=1, seed=5, nsrc=2, nrec=6)
     pb.fwi = dict(dt=2, dx=1, dz=2, x1=pb.ppad - 2, x2=pb.nxt - 3, z1=pb.fpad, z2=pb.nzt - 5)
     src_shot = [0, 1]
@@ -66,7 +74,7 @@ def test_dropin_kernels_fp64_bit_exact(harness, tmp_path, fdorder):
     pb.rec = dict(pb.rec, shot=rec_shot)
     adj = np.random.default_rng(1).standard_normal((6, pb.nt)) * 1e-3
     _write_case(tmp_path, pb, 1, src_shot, rec_shot, adj, snap=0)
-    run(harness, "dropin", tmp_path)
+    run(harness, "dropin", tmp_path, *(["gpu"] if gpu else []))
 
     ref = O.forward(pb, ishot=1)
     n = pb.nzt * pb.nxt
@@ -77,7 +85,7 @@ def test_dropin_kernels_fp64_bit_exact(harness, tmp_path, fdorder):
     assert np.array_equal(rec[mine], ref["rec"][mine])
     assert np.all(rec[other] == -1.0)           # rows of other shots untouched
     fl = np.fromfile(tmp_path / "out_fwd_fields.bin").reshape(6, pb.nzt, pb.nxt)
-    for k, name in enumerate(("vx", "vz", "sxx", "szx", "szz", "We")):
+    for k, name in enumerate(fields):
         assert np.array_equal(fl[k], ref[name]), name
     assert np.array_equal(np.fromfile(tmp_path / "out_fwd_mem.bin"), ref["mem"])
     store = np.fromfile(tmp_path / "out_store.bin").reshape(ref["store"].shape)
@@ -92,7 +100,7 @@ def test_dropin_kernels_fp64_bit_exact(harness, tmp_path, fdorder):
     for k, name in enumerate(("grad_lam", "grad_mu", "grad_rho")):
         assert np.array_equal(g[k], a[name]), name
     fa = np.fromfile(tmp_path / "out_adj_fields.bin").reshape(6, pb.nzt, pb.nxt)
-    for k, name in enumerate(("vx", "vz", "sxx", "szx", "szz", "We")):
+    for k, name in enumerate(fields):
         assert np.array_equal(fa[k], a[name]), name
 
 

@@ -194,3 +194,44 @@ def test_tile_heights_refused():
         plan = plan_of(pb, prec)
         with pytest.raises(P.PsvfwiError, match="tile height"):
             plan.set_tile(bad)
+
+
+def test_results_dropped_when_geometry_changes():
+    """A full-store forward followed by the autotuner (which runs its candidates in the
+    plan's own state, store and gradient buffers), or by a new window or batch, leaves no
+    result readable: the getters refuse with PSVFWI_E_STATE instead of returning the
+    autotuner's values or reading past buffers sized for the old geometry."""
+    from cuda_fwi_amd import psvfwi as P
+    print("stale results: autotune / set_window / set_shots", flush=True)
+    pb = RC.small_problem(nx=40, nz=30, npml=6, nt=60, fdorder=4, fsurf=1, seed=2, nsrc=1, nrec=4)
+    pb.fwi = RC.full_window(pb, dt=2)
+
+    def state_error(fn):
+        with pytest.raises(P.PsvfwiError) as e:
+            fn()
+        assert e.value.status == -5, str(e.value)
+
+    plan = plan_of(pb, "fp32")
+    plan.forward(energy=False, gradient=True)
+    plan.store(0)                        # held
+    plan.autotune(4)
+    state_error(lambda: plan.store(0))
+    state_error(lambda: plan.records(0))
+    state_error(lambda: plan.field("vx", 0))
+
+    plan.forward(energy=False, gradient=True)
+    plan.misfit([np.zeros((4, pb.nt))])
+    plan.adjoint(energy=False, gradient=True)
+    plan.gradient(0)                     # held
+    w = dict(pb.fwi, x2=pb.fwi["x2"] - 7, z2=pb.fwi["z2"] - 5)
+    plan.set_window(w)
+    state_error(lambda: plan.gradient(0))
+    state_error(lambda: plan.store(0))
+    state_error(lambda: plan.adjoint(energy=False, gradient=True))
+
+    plan.set_window(pb.fwi)
+    plan.forward(energy=False, gradient=True)
+    plan.set_shots([shots_of(pb), shots_of(pb)])
+    state_error(lambda: plan.store(1))
+    state_error(lambda: plan.records(1))
+    plan.close()

@@ -209,3 +209,15 @@ def test_config_errors(harness, tmp_path):
     p.write_text(REF_LIKE_INI + "\n[mi355x]\nprecision = fp16\n")
     with pytest.raises(RuntimeError, match="precision"):
         run(harness, "config", p, tmp_path / "o.txt", cwd=tmp_path)
+
+
+@pytest.mark.parametrize("L2,steps", [((3.0, 2.0, 2.5), (0.0, 1.0, 2.0)),
+                                      ((1.0, 0.7, 0.8), (0.0, 0.01, 0.03)),
+                                      ((5.0, 4.0, 4.5), (0.0, 2.0, 3.0))])
+def test_calc_opt_step_dropin(harness, tmp_path, L2, steps):
+    """calc_opt_step (CUDA_week2/fd_kernel.cuh:72) through fd_kernel.hpp: the vertex of the
+    parabola through the three trial misfits, against numpy's quadratic fit."""
+    out = tmp_path / "opt.txt"
+    run(harness, "opt", *[repr(v) for v in L2 + steps], out)
+    c = np.polyfit(steps, L2, 2)
+    assert np.isclose(float(out.read_text()), -c[1] / (2 * c[0]), rtol=1e-12)
