@@ -30,4 +30,12 @@ struct CopyList {
 };
 hipError_t launch_copies(const CopyList& l, hipStream_t s);
 
+// Device scratch of the no-allocation misfit (psvfwi_misfit_noalloc): a static
+// device array of kNoAllocBytes per GPU, held until the process ends.  Returns
+// its address on the current device (resolved once per device with
+// hipGetSymbolAddress; the array's host-side name is not a device address).
+// Every call on one GPU shares it: calls that overlap on one GPU race.
+constexpr size_t kNoAllocBytes = 544768;  // 1024 shots x (64 partials + 1 misfit) doubles + flag
+hipError_t noalloc_scratch(void** ptr);
+
 }  // namespace psv

@@ -2,9 +2,35 @@
 // residual, FP64 shot-sum of the gradients, fill and strided copies.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "psv_aux.h"
 
 namespace psv {
+
+// Scratch of the no-allocation entry point: partial sums, per-shot misfits and
+// the range flag of psvfwi_misfit_noalloc.  One copy per GPU, at its own address.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+hipError_t noalloc_scratch(void** ptr)
+{
+    constexpr int kMaxDevices = 64;
+    static void* addr[kMaxDevices] = {};
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
+}
 
 constexpr int kRedThreads = 256;
 

@@ -133,6 +133,7 @@ struct psvfwi_plan {
     virtual psvfwi_status forward(int) = 0;
     virtual psvfwi_status set_observed(const double*) = 0;
     virtual psvfwi_status misfit(const double*, double*) = 0;
+    virtual psvfwi_status misfit_noalloc(double*) = 0;
     virtual psvfwi_status set_adjoint_sources(int, const double*) = 0;
     virtual psvfwi_status set_store(int, const double*) = 0;
     virtual psvfwi_status adjoint(int) = 0;
@@ -677,6 +678,41 @@ public:
         if (flag) return fail(PSVFWI_E_NONFINITE, "misfit: |residual| > 1e10 (the reference aborts here)");
         for (int i = 0; i < nslots_; i++)
             if (!std::isfinite(out[i])) return fail(PSVFWI_E_NONFINITE, "misfit: non-finite misfit");
+        return PSVFWI_OK;
+    }
+
+    // misfit() on the resident observed traces, with the partial sums, per-shot misfits and
+    // range flag in the library's static scratch instead of plan buffers: no device
+    // allocation (the kernels are misfit()'s, so the results are the same bit for bit)
+    psvfwi_status misfit_noalloc(double* out) override
+    {
+        if (!fwd_done_) return fail(PSVFWI_E_STATE, "misfit_noalloc: run forward first");
+        if (!out) return fail(PSVFWI_E_ARG, "misfit_noalloc: null argument");
+        if (!have_obs_) return fail(PSVFWI_E_STATE, "misfit_noalloc: no observed traces (psvfwi_set_observed)");
+        const int nchunk = 64;
+        if (nslots_ > PSVFWI_NOALLOC_MAX_SHOTS)
+            return fail(PSVFWI_E_ARG, "misfit_noalloc: more than PSVFWI_NOALLOC_MAX_SHOTS shots in the batch");
+        const size_t need = ((size_t)nslots_ * (nchunk + 1)) * sizeof(double) + sizeof(int);
+        if (need > psv::kNoAllocBytes) return fail(PSVFWI_E_ARG, "misfit_noalloc: batch exceeds the scratch");
+        void* base = nullptr;
+        PSV_HIP(psv::noalloc_scratch(&base));
+        double* partial = static_cast<double*>(base);
+        double* mis = partial + (size_t)nslots_ * nchunk;
+        int* flag = reinterpret_cast<int*>(mis + nslots_);
+        // the flag starts at zero on every call (a fill kernel, not a memset)
+        PSV_HIP(psv::launch_fill<int>(flag, 1, 0, stream_));
+        PSV_HIP(psv::launch_misfit<T>(rec_.p, obs_.p, adj_sig_.p, rec_out_off_.p, rec_len_.p, nslots_,
+                                      nchunk, 0.5 * g_.dt, partial, mis, flag, stream_));
+        std::vector<double> m((size_t)nslots_);
+        int hflag = 0;
+        PSV_HIP(hipMemcpyAsync(m.data(), mis, m.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        PSV_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        PSV_HIP(hipStreamSynchronize(stream_));
+        for (int i = 0; i < nslots_; i++) out[i] = m[(size_t)slot_of_shot_[(size_t)i]];
+        std::fill(adj_ready_.begin(), adj_ready_.end(), 1);
+        if (hflag) return fail(PSVFWI_E_NONFINITE, "misfit_noalloc: |residual| > 1e10 (the reference aborts here)");
+        for (int i = 0; i < nslots_; i++)
+            if (!std::isfinite(out[i])) return fail(PSVFWI_E_NONFINITE, "misfit_noalloc: non-finite misfit");
         return PSVFWI_OK;
     }
 
@@ -2147,6 +2183,12 @@ psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* m
 {
     PSV_PLAN(plan);
     return plan->misfit(observed, misfit_out);
+}
+
+psvfwi_status psvfwi_misfit_noalloc(psvfwi_plan* plan, double* misfit_out)
+{
+    PSV_PLAN(plan);
+    return plan->misfit_noalloc(misfit_out);
 }
 
 psvfwi_status psvfwi_set_adjoint_sources(psvfwi_plan* plan, int shot, const double* signal)

@@ -37,7 +37,8 @@ EXPORTS = ("psvfwi_last_error", "psvfwi_plan_create", "psvfwi_plan_destroy", "ps
            "psvfwi_snapshot_count", "psvfwi_get_snapshot", "psvfwi_set_graph_steps",
            "psvfwi_set_gradient_mode", "psvfwi_get_coefficient_gradient", "psvfwi_get_model_gradient",
            "psvfwi_chain_gradient", "psvfwi_random_boundary", "psvfwi_set_shot_order",
-           "psvfwi_get_shot_order", "psvfwi_set_launch_group", "psvfwi_get_launch_group")
+           "psvfwi_get_shot_order", "psvfwi_set_launch_group", "psvfwi_get_launch_group",
+           "psvfwi_misfit_noalloc")
 
 
 class PsvfwiError(RuntimeError):
@@ -238,6 +239,13 @@ class Plan:
         else:
             obs = self._concat(observed)
             _check(lib().psvfwi_misfit(self._h, _dptr(obs), _dptr(out)))
+        return out
+
+    def misfit_noalloc(self):
+        """misfit() on the resident observed traces with no device allocation (the
+        library's static scratch; calls that overlap on one GPU race)."""
+        out = np.zeros(len(self._shots))
+        _check(lib().psvfwi_misfit_noalloc(self._h, _dptr(out)))
         return out
 
     def set_adjoint_sources(self, shot, signal):

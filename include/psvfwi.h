@@ -180,6 +180,18 @@ psvfwi_status psvfwi_set_observed(psvfwi_plan* plan, const double* observed);
  * observed: as in psvfwi_set_observed, or NULL to use the traces set there. */
 psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out);
 
+/* psvfwi_misfit without device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It reads the observed traces resident from
+ * psvfwi_set_observed (an allocating setup call) and computes what psvfwi_misfit(plan, NULL,
+ * misfit_out) computes, bit for bit, with its partial sums, per-shot misfits and range flag
+ * in a static device array of the library (about 0.5 MB per GPU, held until the process
+ * ends) instead of plan buffers.  At most PSVFWI_NOALLOC_MAX_SHOTS shots per batch
+ * (PSVFWI_E_ARG beyond).  Every plan on one GPU shares that array, which keeps the last
+ * call's contents: calls that overlap on one GPU race. */
+#define PSVFWI_NOALLOC_MAX_SHOTS 1024
+psvfwi_status psvfwi_misfit_noalloc(psvfwi_plan* plan, double* misfit_out);
+
 /* Explicit adjoint sources for shot i: [nrec_i][nt_i], injected at the shot's
  * receivers (the reference passes the receivers as sources,
  * CUDA_week2/fd_wave_model.cu:848-859). */
