@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "psv_dfloat.h"
+
 namespace psv {
 
 template <typename T>

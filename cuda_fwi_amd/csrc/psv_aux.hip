@@ -166,5 +166,11 @@ template hipError_t launch_grad_sum<double>(const double*, size_t, const int*, i
 template hipError_t launch_fill<float>(float*, size_t, float, hipStream_t);
 template hipError_t launch_fill<double>(double*, size_t, double, hipStream_t);
 template hipError_t launch_fill<int>(int*, size_t, int, hipStream_t);
+template hipError_t launch_misfit<dfloat>(const dfloat*, const double*, dfloat*, const size_t*,
+                                          const size_t*, int, int, double, double*, double*, int*,
+                                          hipStream_t);
+template hipError_t launch_grad_sum<dfloat>(const dfloat*, size_t, const int*, int, size_t, double*,
+                                            hipStream_t);
+template hipError_t launch_fill<dfloat>(dfloat*, size_t, dfloat, hipStream_t);
 
 }  // namespace psv

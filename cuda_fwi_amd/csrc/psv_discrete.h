@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "psv_dfloat.h"
+
 namespace psv {
 
 // workgroup shape of the reverse-step kernels: 64 x 4 cells

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "psv_dfloat.h"
+
 namespace psv {
 
 constexpr int kTX = 64;        // tile width: one wave64 row
@@ -118,5 +120,13 @@ constexpr int kMaxTz = 32;
 // or 16.  Returns hipErrorInvalidValue for other tile heights.
 template <typename T>
 hipError_t launch_fused(const StepParams<T>& p, int order, int mode, int tz, hipStream_t s);
+
+// One step as a single launch in which each workgroup marches down a segment
+// of 64 columns x `rows` rows (32 or 64), keeping the rows it needs in LDS
+// rings, followed by the point-source / receiver launch (and the explosive
+// one when needed); reads vx.. and writes vx_w.. like launch_fused.  Returns
+// hipErrorInvalidValue for other segment heights.
+template <typename T>
+hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, hipStream_t s);
 
 }  // namespace psv

@@ -290,11 +290,12 @@ public:
         return PSVFWI_OK;
     }
 
-    // tz > 0: split step kernels with 64 x tz tiles; tz < 0: the fused step
-    // kernel with 64 x |tz| tiles
+    // tz > 0: split step kernels with 64 x tz tiles; tz = -8, -16: the fused step
+    // kernel with 64 x |tz| tiles; tz = -32, -64: the z-marching fused step on
+    // 64-column segments of |tz| rows
     static bool tile_ok(int tz)
     {
-        if (tz < 0) return tz == -8 || tz == -16;
+        if (tz < 0) return tz == -8 || tz == -16 || tz == -32 || tz == -64;
         return tz == 4 || tz == 8 || tz == 16 || (tz == 32 && sizeof(T) == 4);
     }
     int tz_abs() const { return tz_ < 0 ? -tz_ : tz_; }
@@ -401,7 +402,7 @@ public:
             *out = ms;
             return PSVFWI_OK;
         };
-        const int cand[kNumCand] = {4, 8, 16, 32, -8, -16};
+        const int cand[kNumCand] = {4, 8, 16, 32, -8, -16, -32, -64};
         float best_ms = 1e30f;
         int best_tz = tz_;
         for (int c = 0; c < kNumCand; c++) {
@@ -1784,7 +1785,8 @@ private:
         p.mvzx_w = alt.ms[1].p;
         p.mvxz_w = alt.ms[2].p;
         p.mvzz_w = alt.ms[3].p;
-        PSV_HIP(psv::launch_fused<T>(p, g_.fdorder, mode, tz_abs(), stream_));
+        if (tz_abs() >= 32) PSV_HIP(psv::launch_march<T>(p, g_.fdorder, mode, tz_abs(), stream_));
+        else PSV_HIP(psv::launch_fused<T>(p, g_.fdorder, mode, tz_abs(), stream_));
         if (e_mid) PSV_HIP(hipEventRecord(e_mid, stream_));
         for (int f = 0; f < 5; f++) swap_buf(st.f[f], alt.f[f]);
         for (int m = 0; m < 4; m++) swap_buf(st.ms[m], alt.ms[m]);
@@ -2006,8 +2008,8 @@ private:
     bool have_obs_ = false;
     double last_ms_ = 0;
     long launches_ = 0;
-    static constexpr int kNumCand = 6;
-    float tune_ms_[kNumCand] = {0, 0, 0, 0, 0, 0};
+    static constexpr int kNumCand = 8;
+    float tune_ms_[kNumCand] = {};
     hipEvent_t sample_ev_[kMaxSamples][3] = {};
     int samp_kind_[kMaxSamples] = {};
     int nsamp_ = 0;
@@ -2044,7 +2046,8 @@ psvfwi_status psvfwi_plan_create(const psvfwi_grid_desc* g, psvfwi_precision pre
     if (g->npml > 0 && (g->nxt < 2 * ppad || g->nzt < (g->fsurf ? ppad + fpad : 2 * ppad)))
         return fail(PSVFWI_E_ARG, "plan_create: CPML wider than the grid");
     if (!(g->dt > 0) || !(g->dx > 0) || !(g->dz > 0)) return fail(PSVFWI_E_ARG, "plan_create: dt, dx, dz must be > 0");
-    if (prec != PSVFWI_FP32 && prec != PSVFWI_FP64) return fail(PSVFWI_E_ARG, "plan_create: unknown precision");
+    if (prec != PSVFWI_FP32 && prec != PSVFWI_FP64 && prec != PSVFWI_FP32X2)
+        return fail(PSVFWI_E_ARG, "plan_create: unknown precision");
     {
         // the kernels address a shot's full-grid arrays with 32-bit byte offsets
         const long long tiles_x = (g->nxt + psv::kTX - 1) / psv::kTX;
@@ -2056,6 +2059,10 @@ psvfwi_status psvfwi_plan_create(const psvfwi_grid_desc* g, psvfwi_precision pre
     std::unique_ptr<psvfwi_plan> p;
     if (prec == PSVFWI_FP32) {
         auto q = std::make_unique<Plan<float>>(*g);
+        PSV_TRY(q->init());
+        p = std::move(q);
+    } else if (prec == PSVFWI_FP32X2) {
+        auto q = std::make_unique<Plan<psv::dfloat>>(*g);
         PSV_TRY(q->init());
         p = std::move(q);
     } else {

@@ -20,7 +20,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpsvfwi.so")
 
-FP32, FP64 = 0, 1
+FP32, FP64, FP32X2 = 0, 1, 2
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32x2": FP32X2}
 ENERGY, GRADIENT = 1, 2
 STORE_FULL, STORE_CHECKPOINT = 0, 1
 FIELDS = {"vx": 0, "vz": 1, "sxx": 2, "szx": 3, "szz": 4, "We": 5, "ux": 6, "uz": 7,
@@ -118,7 +119,9 @@ class Plan:
         self.grid = dict(nzt=nzt, nxt=nxt, fdorder=fdorder, npml=npml, fsurf=int(fsurf),
                          dt=dt, dx=dx, dz=dz)
         self._h = C.c_void_p()
-        self.prec = FP32 if precision in ("fp32", FP32) else FP64
+        if precision not in PRECISIONS and precision not in PRECISIONS.values():
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.prec = PRECISIONS.get(precision, precision)
         _check(lib().psvfwi_plan_create(C.byref(g), self.prec, C.byref(self._h)))
         self._shots = []
         self.window = None

@@ -39,7 +39,13 @@ typedef enum psvfwi_status {
     PSVFWI_E_NOMEM = -6       /* device allocation failed */
 } psvfwi_status;
 
-typedef enum psvfwi_precision { PSVFWI_FP32 = 0, PSVFWI_FP64 = 1 } psvfwi_precision;
+/* Arithmetic of a plan's kernels.  PSVFWI_FP32X2 (no counterpart in the reference, which
+ * computes in FP64, CUDA_week2/globvar.cuh:16) is compensated FP32: every value is an
+ * unevaluated sum of two floats and every operation a double-word algorithm built from FP32
+ * operations (about 48 significant bits, 8 bytes per value).  It is for cases whose
+ * residual is far below the wavefield (a small anomaly), where plain FP32 rounds the
+ * difference of two forward runs away. */
+typedef enum psvfwi_precision { PSVFWI_FP32 = 0, PSVFWI_FP64 = 1, PSVFWI_FP32X2 = 2 } psvfwi_precision;
 
 /* flags of psvfwi_forward / psvfwi_adjoint */
 #define PSVFWI_ENERGY 1   /* accumulate the energy weight We (fd_wave.cuh:14-29) */
@@ -117,9 +123,12 @@ psvfwi_status psvfwi_set_shots(psvfwi_plan* plan, int nshots, const psvfwi_shot_
 
 /* Step kernel and tile height.  tz > 0: two launches per step (stress, then
  * velocity, in place) on 64 x tz tiles, tz = 4, 8, 16 or 32 (FP64: up to 16);
- * tz < 0: one fused launch per step on 64 x |tz| tiles, tz = -8 or -16
- * (ping-pong state, stress ring recomputed); 0 = default (8).  Every choice
- * gives the same results bit for bit. */
+ * tz = -8 or -16: one fused launch per step on 64 x |tz| tiles (ping-pong
+ * state, stress ring recomputed); tz = -32 or -64: one launch per step in
+ * which each workgroup marches down a segment of 64 columns x |tz| rows,
+ * every stress row computed once (ping-pong state), plus a small launch for
+ * the point sources and receivers; 0 = default (8).  Every choice gives the
+ * same results bit for bit. */
 psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz);
 
 /* Autotuner: times `steps` forward steps of the current batch with every

@@ -4,19 +4,21 @@ end against the oracle: 201 x 201 cells, dx 5 m, npml 20, free surface, a
 at (100, 2), 101 receivers along the surface, dt 0.5 ms, nt 2000.
 
 - order 4 with the explosive source: the forward of the anomaly model, FP64
-  bit for bit and FP32 within 1e-5 relative L2 of the FP64 oracle;
+  bit for bit; FP32 and compensated FP32 (fp32x2) within 1e-5 relative L2 of
+  the FP64 oracle;
 - observed = forward of the anomaly model, synthetic = forward of the
   homogeneous background; one forward + misfit + adjoint + gradient over the
   model window (time decimation 10), orders 4 (explosive source) and 2 (vz
-  force, the reference's own source type): gradients FP64 bit for bit and
-  FP32 bit for bit against the FP32 oracle.  Against the FP64 gradient the
-  FP32 one is limited by FP32 itself, not by the kernels: the anomaly is
-  small (20 m), its scattered field is 1e-3 to 1e-4 of the direct wave, so
-  FP32 traces resolve the residual only to about 3e-3, and the rounding noise
-  of the direct wave correlates with the strong forward field near the source
-  and receivers.  Measured: 1.7e-2 to 3.7e-2 relative L2 with the explosive
-  source; the test bounds it at 0.1 and the direction (correlation) at
-  0.999.
+  force, the reference's own source type): FP64 bit for bit, FP32 bit for bit
+  against the FP32 oracle, and compensated FP32 within the stated tolerances
+  of the FP64 oracle: seismograms 1e-5 and gradients 1e-4 relative L2.
+- Plain FP32 does not meet the gradient tolerance on this case, and the test
+  that asserts it is an expected failure: the anomaly is small (20 m), its
+  scattered field is 1e-3 to 1e-4 of the direct wave, so FP32 synthetics
+  resolve the residual only to about 3e-3, and the rounding noise of the
+  direct wave correlates with the strong forward field near the source and
+  receivers (measured 1.7e-2 to 3.7e-2 relative L2).  The kernels are not the
+  cause (they equal the FP32 oracle bit for bit); compensated FP32 removes it.
 """
 import numpy as np
 import pytest
@@ -59,43 +61,73 @@ def test_c1_forward_explosive_order4():
     assert np.array_equal(p64.records(), ref["rec"])
     for k in FIELDS:
         assert np.array_equal(p64.field(k), ref[k]), k
-    p32 = plan_of(pb, "fp32")
-    p32.forward(energy=True)
-    e = rel_l2(p32.records(), ref["rec"])
-    print(f"  fp32 seismogram rel L2 vs fp64 oracle: {e:.2e}")
-    assert e < 1e-5
+    for prec in ("fp32", "fp32x2"):
+        p = plan_of(pb, prec)
+        p.forward(energy=True)
+        e = rel_l2(p.records(), ref["rec"])
+        print(f"  {prec} seismogram rel L2 vs fp64 oracle: {e:.2e}")
+        assert e < 1e-5
 
 
-@pytest.mark.parametrize("fdorder,comp", [(4, 4), (2, 2)])
-def test_c1_fwd_adj_gradient(fdorder, comp):
-    print(f"201 x 201 fwd + adj + gradient, order {fdorder}, source comp {comp}", flush=True)
+def _c1_gradients(fdorder, comp, precs):
+    """Oracle FP64 (forward, residual, gradient) and each precision's misfit, traces and
+    gradient on the GPU."""
     obs = O.forward(_problem(fdorder, comp))["rec"]
     pb = _problem(fdorder, comp, background=True, fwi_dt=10)
     f = O.forward(pb)
     m, res, _ = O.l2_adjoint_sources(pb.dt, f["rec"].copy(), obs)
     a = O.adjoint(pb, pb.rec, res, f["store"])
-    p64 = plan_of(pb, "fp64")
-    p64.forward(gradient=True)
-    mis = p64.misfit([obs])
-    assert abs(mis[0] - m) <= 1e-12 * m
-    p64.adjoint(gradient=True)
-    g64 = p64.gradient(0)
-    for got, k in zip(g64, ("grad_lam", "grad_mu", "grad_rho")):
+    out = {}
+    for prec in precs:
+        p = plan_of(pb, prec)
+        p.forward(gradient=True)
+        mis = p.misfit([obs])[0]
+        rec = p.records()
+        p.adjoint(gradient=True)
+        out[prec] = dict(misfit=mis, rec=rec, grad=p.gradient(0))
+        p.close()
+    return pb, obs, f, m, res, a, out
+
+
+GRADS = ("grad_lam", "grad_mu", "grad_rho")
+
+
+@pytest.mark.parametrize("fdorder,comp", [(4, 4), (2, 2)])
+def test_c1_fwd_adj_gradient(fdorder, comp):
+    print(f"201 x 201 fwd + adj + gradient, order {fdorder}, source comp {comp}", flush=True)
+    pb, obs, f, m, res, a, out = _c1_gradients(fdorder, comp, ("fp64", "fp32", "fp32x2"))
+    o64 = out["fp64"]
+    assert abs(o64["misfit"] - m) <= 1e-12 * m
+    for got, k in zip(o64["grad"], GRADS):
         assert np.abs(a[k]).max() > 0
         assert np.array_equal(got, a[k]), k
-    p32 = plan_of(pb, "fp32")
-    p32.forward(gradient=True)
-    p32.misfit([obs])
-    p32.adjoint(gradient=True)
-    g32 = p32.gradient(0)
+    # plain FP32 against the FP32 oracle: bit for bit
     f32 = O.forward(pb, prec="f32")
     res32 = (f32["rec"].astype(np.float64) - obs).astype(np.float32)
     a32 = O.adjoint(pb, pb.rec, res32, f32["store"], prec="f32")
-    for got, k in zip(g32, ("grad_lam", "grad_mu", "grad_rho")):
+    for got, k in zip(out["fp32"]["grad"], GRADS):
         assert np.array_equal(got, a32[k].astype(np.float64)), k
-    eres = rel_l2(res32, res)
-    errs = [rel_l2(x, y) for x, y in zip(g32, g64)]
-    corr = [float(np.corrcoef(x.ravel(), y.ravel())[0, 1]) for x, y in zip(g32, g64)]
-    print(f"  fp32 residual rel L2 {eres:.2e}; fp32 gradients vs fp64: rel L2 {[f'{e:.2e}' for e in errs]}, "
-          f"correlation {[f'{c:.6f}' for c in corr]}")
-    assert max(errs) < 0.1 and min(corr) > 0.999
+    for prec in ("fp32", "fp32x2"):
+        o = out[prec]
+        es = rel_l2(o["rec"], f["rec"])
+        eg = [rel_l2(x, a[k]) for x, k in zip(o["grad"], GRADS)]
+        em = abs(o["misfit"] - m) / m
+        print(f"  {prec} vs fp64 oracle: seismogram rel L2 {es:.2e}, misfit rel {em:.2e}, "
+              f"gradients rel L2 {[f'{e:.2e}' for e in eg]}")
+    # compensated FP32: the stated tolerances
+    o = out["fp32x2"]
+    assert rel_l2(o["rec"], f["rec"]) < 1e-5
+    assert max(rel_l2(x, a[k]) for x, k in zip(o["grad"], GRADS)) < 1e-4
+    assert abs(o["misfit"] - m) < 1e-6 * m
+
+
+@pytest.mark.xfail(strict=True, reason="plain FP32 resolves the residual of this 20 m anomaly (1e-3 to 1e-4 "
+                                       "of the direct wave) only to about 3e-3: its gradient differs from "
+                                       "the FP64 one by 1.7e-2 to 3.7e-2 relative L2; compensated FP32 "
+                                       "(fp32x2) meets the 1e-4 tolerance")
+def test_c1_plain_fp32_gradient_within_tolerance():
+    print("201 x 201 gradient, plain FP32 against the 1e-4 tolerance", flush=True)
+    pb, obs, f, m, res, a, out = _c1_gradients(4, 4, ("fp32",))
+    errs = [rel_l2(x, a[k]) for x, k in zip(out["fp32"]["grad"], GRADS)]
+    print(f"  fp32 gradients rel L2 vs fp64 oracle {[f'{e:.2e}' for e in errs]}")
+    assert max(errs) < 1e-4

@@ -190,7 +190,8 @@ def test_tile_heights_refused():
     from cuda_fwi_amd import psvfwi as P
     print("tile heights refused", flush=True)
     pb = _grid_problem(33, 65, 4, 8, 0, nt=10, src_at=(16, 40))
-    for prec, bad in (("fp32", 12), ("fp32", -32), ("fp32", -4), ("fp32", -24), ("fp64", 32)):
+    for prec, bad in (("fp32", 12), ("fp32", -48), ("fp32", -4), ("fp32", -24), ("fp64", 32),
+                      ("fp32", -128)):
         plan = plan_of(pb, prec)
         with pytest.raises(P.PsvfwiError, match="tile height"):
             plan.set_tile(bad)

@@ -1,5 +1,6 @@
-"""The fused step kernel (one launch per time step, ping-pong state,
-set with a negative tile height) against the oracle, bit for bit, on the
+"""The fused step kernels (one launch per time step, ping-pong state, set
+with a negative tile height: -8 / -16 the tile kernel, -32 / -64 the
+z-marching segment kernel) against the oracle, bit for bit, on the
 same cases as the split kernels: orders 2/4 with and without free surface
 (wavefields and CPML memory), gradients with full and decimated windows,
 checkpointed gradients, uneven batches (final state of shots that end
@@ -16,7 +17,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"o{c['fdorder']}fs{c['fsurf']}")
-@pytest.mark.parametrize("prec,tz", [("fp64", -8), ("fp32", -8), ("fp32", -16), ("fp64", -16)])
+@pytest.mark.parametrize("prec,tz", [("fp64", -8), ("fp32", -8), ("fp32", -16), ("fp64", -16),
+                                     ("fp64", -32), ("fp32", -32), ("fp64", -64), ("fp32", -64)])
 def test_fused_forward_bit_exact(case, prec, tz):
     print(f"fused forward {case} {prec} {tz}", flush=True)
     pb = RC.small_problem(nx=90, nz=50, npml=8, nt=260, **case)
@@ -31,13 +33,14 @@ def test_fused_forward_bit_exact(case, prec, tz):
         assert np.array_equal(plan.field(k), om[k].astype(np.float64)), k
 
 
+@pytest.mark.parametrize("tz", [-16, -32])
 @pytest.mark.parametrize("win", list(WINDOWS))
 @pytest.mark.parametrize("fdorder", [2, 4])
-def test_fused_gradient_fp64_bit_exact(fdorder, win):
-    print(f"fused gradient fp64 order {fdorder} window {win}", flush=True)
+def test_fused_gradient_fp64_bit_exact(fdorder, win, tz):
+    print(f"fused gradient fp64 order {fdorder} window {win} tile {tz}", flush=True)
     pb, obs = _fwi_problem(fdorder=fdorder, fwi=WINDOWS[win])
     f, m, res, a = _oracle_gradient(pb, obs)
-    plan = plan_of(pb, "fp64", tz=-16)
+    plan = plan_of(pb, "fp64", tz=tz)
     plan.forward(energy=True, gradient=True)
     assert np.array_equal(plan.store(), f["store"])
     plan.misfit([obs])
@@ -48,10 +51,11 @@ def test_fused_gradient_fp64_bit_exact(fdorder, win):
         assert np.array_equal(plan.field(k), a[k]), k
 
 
-def test_fused_checkpoint_gradient_equals_full_store():
-    print("fused checkpoint vs store", flush=True)
+@pytest.mark.parametrize("tz", [-8, -64])
+def test_fused_checkpoint_gradient_equals_full_store(tz):
+    print(f"fused checkpoint vs store, tile {tz}", flush=True)
     pb, obs = _fwi_problem(nt=333, fwi=WINDOWS["decimated"])
-    plan = plan_of(pb, "fp32", tz=-8)
+    plan = plan_of(pb, "fp32", tz=tz)
     plan.forward(gradient=True)
     plan.misfit([obs])
     plan.adjoint(gradient=True)
@@ -71,7 +75,7 @@ def test_fused_checkpoint_gradient_equals_full_store():
             assert np.array_equal(a, b), k
 
 
-@pytest.mark.parametrize("tz", [-16, -8])
+@pytest.mark.parametrize("tz", [-16, -8, -32, -64])
 def test_batched_uneven_shots_variants(tz):
     """Batches with the fused kernel equal single-shot plans of the split
     kernels: shots that end before the longest one keep their own final
@@ -111,15 +115,16 @@ def test_batched_uneven_shots_variants(tz):
             assert np.array_equal(one.field(k), batch.field(k, i)), (i, k)
 
 
-def test_fused_explosive_vx_sources_and_boundary_geometry():
-    print("fused explosive / vx sources, boundary geometry", flush=True)
+@pytest.mark.parametrize("tzs", [(-8, -16), (-32, -64), (-64, -32)])
+def test_fused_explosive_vx_sources_and_boundary_geometry(tzs):
+    print(f"fused explosive / vx sources, boundary geometry, tiles {tzs}", flush=True)
     pb = RC.small_problem(nx=60, nz=60, npml=8, nt=200, fdorder=4, fsurf=0, nsrc=3, src_comp=4)
     pb.src = dict(pb.src)
     pb.src["comp"] = [4, 1, 2]
     pb.src["x"] = [pb.src["x"][0], pb.src["x"][0], pb.src["x"][2]]
     pb.src["z"] = list(pb.src["z"])
     ref = O.forward(pb)
-    plan = plan_of(pb, "fp64", tz=-8)
+    plan = plan_of(pb, "fp64", tz=tzs[0])
     plan.forward()
     assert np.array_equal(plan.records(), ref["rec"])
     for k in FIELDS:
@@ -129,7 +134,7 @@ def test_fused_explosive_vx_sources_and_boundary_geometry():
     pb.src_signal = np.vstack([pb.src_signal, pb.src_signal[:1] * 0.5])
     pb.rec = dict(x=[0, 15, 41], z=[0, 0, 7], comp=[2, 1, 2], shot=[0, 0, 0])
     ref = O.forward(pb)
-    plan = plan_of(pb, "fp64", tz=-16)
+    plan = plan_of(pb, "fp64", tz=tzs[1])
     plan.forward()
     assert np.array_equal(plan.records(), ref["rec"])
     for k in FIELDS:
@@ -147,12 +152,12 @@ def test_fused_snapshots_and_autotune():
     assert np.array_equal(snaps[2], ref["szz"])
     p32 = plan_of(pb, "fp32")
     best = p32.autotune(20)
-    assert best in (4, 8, 16, 32, -8, -16)
+    assert best in (4, 8, 16, 32, -8, -16, -32, -64)
     p32.forward()
     assert np.array_equal(p32.records(), O.forward(pb, prec="f32")["rec"].astype(np.float64))
 
 
-@pytest.mark.parametrize("tz,steps", [(8, 16), (-16, 8), (16, 2)])
+@pytest.mark.parametrize("tz,steps", [(8, 16), (-16, 8), (16, 2), (-64, 8)])
 def test_graph_time_loop_equals_direct_launches(tz, steps):
     """Forward and adjoint replayed as captured graphs (uneven batch: the
     active shots change between graph chunks; the chunk length does not
@@ -215,3 +220,26 @@ def test_shot_orders_give_identical_results():
         for ga, gb in zip(grad, out[0][1]):
             for a, b in zip(ga, gb):
                 assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("fsurf", [0, 1])
+@pytest.mark.parametrize("fdorder", [2, 4])
+def test_march_tall_grid_segments_bit_exact(fdorder, fsurf):
+    """The z-marching kernel on a grid of several 32- and 64-row segments with a short last
+    one, CPML on every side, a source near a segment edge and receivers on the free surface
+    and at the bottom: traces, wavefields and CPML memory against the oracle, bit for bit,
+    and the FP64 gradient of the same case."""
+    print(f"march tall grid order {fdorder} fsurf {fsurf}", flush=True)
+    pb = RC.small_problem(nx=100, nz=150, npml=10, nt=300, fdorder=fdorder, fsurf=fsurf, nsrc=2, nrec=6)
+    pb.src = dict(pb.src, z=[63, 96])
+    pb.rec = dict(pb.rec, z=[pb.fpad, pb.fpad, 64, 64, pb.nzt - pb.ppad - 2, pb.nzt - pb.fpad - 1])
+    ref = O.forward(pb)
+    om = _oracle_mems(pb, ref["mem"])
+    for tz in (-32, -64):
+        plan = plan_of(pb, "fp64", tz=tz)
+        plan.forward(energy=True)
+        assert np.array_equal(plan.records(), ref["rec"]), tz
+        for k in FIELDS:
+            assert np.array_equal(plan.field(k), ref[k]), (tz, k)
+        for k in MEMS:
+            assert np.array_equal(plan.field(k), om[k]), (tz, k)
