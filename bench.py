@@ -1,19 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X P-SV FWI hot path: forward + adjoint + gradient.
 
-Default workload (the single-GPU configuration the metric is quoted on):
-1000 x 500 cells, dx 5 m, order 4 Holberg, 40-cell CPML, 8 flat layers
-(vp 1800-4500 m/s, vs = vp/1.73, Gardner density) with a seeded 3 % Gaussian
-perturbation, one 12 Hz Ricker vz source, 400 receivers, dt 0.4 ms,
-nt 5000, FP32 kernels, gradient window = the model, time decimation 5.
-One "step" = one forward (with the gradient store) + FP64 misfit/residual
-+ adjoint with the fused gradient correlation, all on device-resident
-inputs.  Observed data are produced beforehand from the perturbed model;
-the synthetics come from the unperturbed layers.
+Default workload: the batched one the metric is quoted on.  64 shots on a
+1500 x 600 model (dx 5 m, order 4 Holberg, 40-cell CPML, seeded layers with a
+dipping fault), nt uniform in 3000-8000, 50-400 receivers per shot, FP32
+kernels, gradient window = the model, time decimation 5.  The shots run in
+store groups whose decimated forward stores fit in HBM together; one "step"
+= one group's batched forward (with the store) + FP64 misfit/residual +
+adjoint with the fused gradient correlation, on device-resident models and
+observed traces; steps cycle through the groups.  The same shots are then
+run one shot per launch for the batched / per-shot comparison.  Extra keys:
+the FP64 kernels' rate, the 1000 x 500 single-shot workload, the CPU oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c2|c3mem|c5]
 
-Prints one JSON line.  Gcell-updates/s = 2 * interior cells * nt / time
+Prints one JSON line.  Gcell-updates/s = 2 * interior cells * sum of nt / time
 (forward and adjoint counted, checkpoint recomputation not counted).
 """
 from __future__ import annotations
@@ -187,7 +188,7 @@ def run_c2(args):
     plan = make_plan(w, args.precision, window, args.store, args.interval, args.tile)
     plan.set_graph_steps(args.graph_steps)
     _log("setup done (observed data computed)")
-    esize = 4 if args.precision == "fp32" else 8
+    esize = 8 if args.precision in ("fp64", "fp32x2") else 4
 
     def one_pass():
         plan.forward(energy=False, gradient=True)
@@ -318,7 +319,7 @@ def run_c2_discrete(args):
     _sync()
     T = time.perf_counter() - t0
     cells = w.interior_cells()
-    esize = 4 if args.precision == "fp32" else 8
+    esize = 8 if args.precision in ("fp64", "fp32x2") else 4
     return {
         "metric": METRIC, "value": round(2.0 * cells * w.nt * args.steps / T / 1e9, 3),
         "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -444,12 +445,25 @@ def _groups_for_store(w, window_cells, fwi_dt, esize, budget_bytes):
     return groups
 
 
+def _shot_descs(w, ids):
+    """Padded shot descriptors with their Ricker signals, built once on the host."""
+    from cuda_fwi_amd import psvfwi as P
+    return [w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
+            for i in ids]
+
+
 def run_c4(args):
-    """64 shots of uneven nt and receiver counts: batched plans against the
-    same shots one plan each.  With --store full (default) the shots run in
-    groups whose decimated forward stores fit in HBM together (no
-    recomputation); with --store checkpoint all shots run as one batch with
-    checkpoints every --interval steps and the segments recomputed."""
+    """64 shots of 1500 x 600 cells with uneven nt (3000-8000) and receiver counts (50-400):
+    forward + FP64 misfit + adjoint + gradient, batched, against the same kernels one shot
+    per launch.
+
+    The full decimated forward store of all 64 shots (1.5 TB at fwi_dt 5) exceeds the 288 GB
+    of HBM, so the set runs in store groups (shots sorted by nt, each group's stores within
+    --store-budget-gb).  One benchmark step is one group's batched fwd + misfit + adj +
+    gradient; successive steps cycle through the groups, so --steps at least the number of
+    groups covers all 64 shots.  The shot tables of a group are uploaded inside its step, as
+    they are for each shot of the per-shot comparison.  With --store checkpoint all 64 shots
+    form one group (checkpoints every --interval steps, recomputation not counted)."""
     from cuda_fwi_amd import psvfwi as P
     from cuda_fwi_amd import workloads as WL
     w = WL.c4(nshots=args.nshots)
@@ -459,131 +473,246 @@ def run_c4(args):
         w.nt = max(sh["nt"] for sh in w.shots)
     fwi_dt = args.fwi_dt
     window = w.model_window(dt=fwi_dt)
-    esize = 4 if args.precision == "fp32" else 8
+    prec = args.precision
+    esize = 8 if prec in ("fp64", "fp32x2") else 4
     nwin = w.nx * w.nz
     if args.store == "full":
         groups = _groups_for_store(w, nwin, fwi_dt, esize, args.store_budget_gb * 1e9)
     else:
-        groups = [list(range(len(w.shots)))]
-    _log(f"{len(groups)} group(s) of shots: {[len(g) for g in groups]}")
-
-    # observed traces from the true model (untimed)
+        groups = [sorted(range(len(w.shots)), key=lambda i: -w.shots[i]["nt"])]
+    _log(f"{len(groups)} store group(s) of shots: {[len(g) for g in groups]}")
+    descs = {i: d for i, d in zip(range(len(w.shots)), _shot_descs(w, range(len(w.shots))))}
     lt, mt, rt = w.padded()
-    vp0, vs0, rho0 = w.background
-    lam, mu, rho = w.padded(vp0, vs0, rho0)
+    lam, mu, rho = w.padded(*w.background)
     avg0 = P.average_parameters(mu, rho)
     hc = P.holberg(w.fdorder, w.max_relative_error)
+    pml = P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt)
 
-    def new_plan(shot_ids, store):
+    def new_plan(precision=prec):
         plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt,
-                      dx=w.dx, dz=w.dx, hc=hc, precision=args.precision)
-        plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt))
+                      dx=w.dx, dz=w.dx, hc=hc, precision=precision)
+        plan.set_pml(pml)
         plan.set_window(window)
-        plan.set_store_mode(store, args.interval)
-        plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
-                        for i in shot_ids])
+        plan.set_store_mode(args.store, args.interval)
+        plan.set_graph_steps(args.graph_steps)
         return plan
 
+    # observed traces from the true model (untimed)
     obs = {}
-    plan = new_plan(groups[0], args.store)
-    plan.set_graph_steps(args.graph_steps)
+    plan = new_plan()
     plan.set_model(lt, mt, *P.average_parameters(mt, rt))
-    tile = args.tile if args.tile != 0 else plan.autotune(min(200, w.nt))
-    band = plan.shot_order()
-    if args.launch_group >= 0:
-        plan.set_launch_group(args.launch_group)
-    lgroup = plan.launch_group()
     for g in groups:
-        plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
-                        for i in g])
-        plan.set_tile(tile)
+        plan.set_shots([descs[i] for i in g])
         plan.forward(energy=False)
         for k, i in enumerate(g):
             obs[i] = plan.records(k)
     plan.set_model(lam, mu, *avg0)
-    _log(f"observed traces done, tile {tile}, shot order {band}, launch group {lgroup}")
-
-    def fwd_adj(pl, shot_ids):
-        pl.set_observed([obs[i] for i in shot_ids])
-        pl.forward(energy=False, gradient=True)
-        pl.last_fwd_ms = pl.timing()[0]
-        pl.misfit()
-        pl.adjoint(energy=False, gradient=True)
-
-    # warm-up on the group with the largest store (the plan keeps that
-    # allocation for the others), then the timed batched pass over all groups
-    def store_need(g):
-        return len(g) * (1 + (max(w.shots[i]["nt"] for i in g) - 1) // fwi_dt)
-    gw = max(groups, key=store_need)
-    plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
-                    for i in gw])
+    # kernel, tile, shot order and launch group tuned once on the first (longest) group
+    plan.set_shots([descs[i] for i in groups[0]])
+    tile = args.tile if args.tile != 0 else plan.autotune(min(args.tune_steps, w.nt))
     plan.set_tile(tile)
-    fwd_adj(plan, gw)
-    _sync()
-    t0 = time.perf_counter()
-    for g in groups:
-        tg = time.perf_counter()
-        plan.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])
-                        for i in g])
+    band = plan.shot_order()
+    if args.launch_group >= 0:
+        plan.set_launch_group(args.launch_group)
+    lgroup = plan.launch_group()
+    _log(f"observed traces done; tile {tile}, shot order {band}, launch group {lgroup}")
+
+    kinds = ("forward_stress", "forward_velocity", "adjoint_stress", "adjoint_velocity")
+
+    def group_step(g):
+        """One step: the group's fwd + misfit + adj + gradient; returns loop ms, launches and
+        sampled kernel us of the forward and the adjoint."""
+        plan.set_shots([descs[i] for i in g])
         plan.set_tile(tile)
         plan.set_shot_order(band)
-        ts = time.perf_counter() - tg
-        fwd_adj(plan, g)
-        _sync()
-        _log(f"group of {len(g)}: set_shots {ts * 1e3:.0f} ms, forward {plan.last_fwd_ms:.0f} ms, "
-             f"adjoint {plan.timing()[0]:.0f} ms, wall {(time.perf_counter() - tg) * 1e3:.0f} ms")
-    _sync()
-    tb = time.perf_counter() - t0
-    _log(f"batched pass {tb:.2f} s")
-    del plan
+        plan.set_observed([obs[i] for i in g])
+        plan.forward(energy=False, gradient=True)
+        tf = plan.timing() + plan.kernel_timing("forward")
+        plan.misfit()
+        plan.adjoint(energy=False, gradient=True)
+        ta = plan.timing() + plan.kernel_timing("adjoint")
+        return tf, ta
 
-    # the same kernels, one shot per plan, on a sample of the shots
-    tp = 0.0
-    sample = list(range(min(args.per_shot_sample, len(w.shots))))
-    p1 = new_plan([sample[0]], args.store)
-    p1.set_graph_steps(args.graph_steps)
+    # setup: one pass of the group with the largest store, so that the store allocation (which
+    # only grows) is made before the timed steps
+    def store_need(g):
+        return len(g) * (1 + (max(w.shots[i]["nt"] for i in g) - 1) // fwi_dt)
+    group_step(max(groups, key=store_need))
+    for i in range(args.warmup):
+        group_step(groups[i % len(groups)])
+        _log(f"warmup step {i} done")
+    _sync()
+    t0 = time.perf_counter()
+    shots_done = cell_steps = 0
+    loop_ms = {"forward": 0.0, "adjoint": 0.0}
+    nlaunch = {"forward": 0, "adjoint": 0}
+    samples = {k: [] for k in kinds}
+    alg_kernel = {k: 0.0 for k in kinds}
+    kb = alg_bytes(w, nwin, fwi_dt, esize)
+    for i in range(args.steps):
+        g = groups[i % len(groups)]
+        ts = time.perf_counter()
+        (fms, fl, fs_us, fv_us, _), (ams, al, as_us, av_us, _) = group_step(g)
+        nts = sum(w.shots[j]["nt"] for j in g)
+        shots_done += len(g)
+        cell_steps += nts * w.interior_cells()
+        loop_ms["forward"] += fms
+        loop_ms["adjoint"] += ams
+        nlaunch["forward"] += fl
+        nlaunch["adjoint"] += al
+        for k, v in zip(kinds, (fs_us, fv_us, as_us, av_us)):
+            samples[k].append(v)
+            alg_kernel[k] += kb[k] * nts
+        _log(f"timed step {i}: group of {len(g)} shots, forward {fms:.0f} ms, adjoint {ams:.0f} ms, "
+             f"wall {(time.perf_counter() - ts) * 1e3:.0f} ms")
+    _sync()
+    T = time.perf_counter() - t0
+
+    # dominant kernel: each kind's share of its loop time in the ratio of the sampled
+    # event-bracketed durations; per launch = that time / launches, algorithmic bytes per
+    # launch = the kernel's bytes summed over all shot-steps / launches
+    kus_total = {}
+    for kind in ("forward", "adjoint"):
+        a = float(np.mean(samples[kind + "_stress"]))
+        b = float(np.mean(samples[kind + "_velocity"]))
+        kus_total[kind + "_stress"] = loop_ms[kind] * 1e3 * a / (a + b)
+        kus_total[kind + "_velocity"] = loop_ms[kind] * 1e3 * b / (a + b)
+    launches = {k: nlaunch[k.split("_")[0]] for k in kinds}
+    kname = max(kus_total, key=lambda k: kus_total[k])
+    avg_us = kus_total[kname] / launches[kname]
+    bytes_per_launch = alg_kernel[kname] / launches[kname]
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    alg_total = sum(alg_kernel.values())
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pmc_traffic_c4.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get(kname)
+    batched_rate = shots_done / T
+    batched_gcell = 2.0 * cell_steps / T / 1e9
+    del plan
+    _log(f"batched: {batched_rate:.3f} shots/s, {batched_gcell:.2f} Gcell-updates/s")
+
+    # the same kernels, one shot per plan, over all shots (shot tables uploaded per shot)
+    p1 = new_plan()
     p1.set_model(lam, mu, *avg0)
+    p1.set_shots([descs[0]])
     p1.set_tile(tile)
-    fwd_adj(p1, [sample[0]])  # warm-up
-    for i in sample:
-        p1.set_shots([w.padded_shot(w.shots[i], P.ricker(w.shots[i]["nt"], w.dt, w.shots[i]["amp"], w.fc)[None])])
+    p1.set_observed([obs[0]])
+    p1.forward(energy=False, gradient=True)   # warm-up
+    p1.misfit()
+    p1.adjoint(energy=False, gradient=True)
+    per = list(range(len(w.shots))) if args.per_shot_sample <= 0 else list(range(min(args.per_shot_sample,
+                                                                                      len(w.shots))))
+    _sync()
+    t1 = time.perf_counter()
+    per_cells = 0
+    for i in per:
+        p1.set_shots([descs[i]])
         p1.set_tile(tile)
+        p1.set_observed([obs[i]])
+        p1.forward(energy=False, gradient=True)
+        p1.misfit()
+        p1.adjoint(energy=False, gradient=True)
+        per_cells += w.shots[i]["nt"] * w.interior_cells()
+    _sync()
+    Tp = time.perf_counter() - t1
+    del p1
+    per_rate = len(per) / Tp
+    per_gcell = 2.0 * per_cells / Tp / 1e9
+    _log(f"per-shot: {per_rate:.3f} shots/s, {per_gcell:.2f} Gcell-updates/s over {len(per)} shots")
+
+    line = {
+        "metric": METRIC, "value": round(batched_gcell, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(T / args.steps * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp32fma": "f32", "fp64": "f64", "fp32x2": "f32x2"}[prec],
+        "data": "synthetic: seeded layered model with a dipping fault (observed), the same smoothed "
+                "over 50 m (synthetic)",
+        "config": {"workload": f"1500x600 dx5 order-4 npml40, {len(w.shots)} shots, nt U[3000,8000], "
+                               "receivers U[50,400], dt 0.5 ms, fwd+adj+grad; one step = one store group "
+                               "of the shot set, steps cycle through the groups",
+                   "padded_grid": [w.nzt, w.nxt], "interior_cells": w.interior_cells(),
+                   "store_groups": [len(g) for g in groups],
+                   "gradient_store": (f"full decimated store, fwi_dt {fwi_dt}, groups sized to "
+                                      f"{args.store_budget_gb} GB of store" if args.store == "full" else
+                                      f"checkpoint every {args.interval} steps, segments recomputed "
+                                      "(recomputation not counted)"),
+                   "tile": tile, "step_kernel": ("split (stress, velocity)" if tile > 0 else
+                                                 "fused tile" if tile > -32 else "z-marching segments"),
+                   "shot_order": "shot-major" if band == 0 else f"bands of {band} tile rows",
+                   "launch_group": lgroup or "whole group", "graph_steps": args.graph_steps},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[kname], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this kernel, "
+                                        "scaled by the copy calibration, from profiles/pmc_traffic_c4.json "
+                                        "(a separate profiled run, not this one)") if traffic else None,
+                     "alg_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_us, 2),
+                     "launches": launches[kname],
+                     "kernel_us_per_launch": {k: round(kus_total[k] / launches[k], 2) for k in kinds}},
+        "hbm": {"alg_bytes": int(alg_total), "alg_GBps": round(alg_total / T / 1e9, 1),
+                "frac_of_peak": round(alg_total / T / 1e9 / HBM_PEAK_GBS, 4),
+                "note": "algorithmic bytes of forward + adjoint + gradient of every timed step over the "
+                        "wall time of the timed steps"},
+        "batched_shots_per_s": round(batched_rate, 3),
+        "per_shot_shots_per_s": round(per_rate, 3),
+        "per_shot_gcell_updates_per_s": round(per_gcell, 3),
+        "per_shot_shots": len(per),
+        "batched_over_per_shot": round(batched_rate / per_rate, 3),
+        "batched_over_per_shot_per_cell_step": round(batched_gcell / per_gcell, 3),
+    }
+    return line
+
+
+def c4_extras(args, line):
+    """Keys beside the batched line: the FP64 build's rate on one store group, the
+    1000 x 500 single-shot workload, and the CPU baseline on a sample of a C4 shot."""
+    from cuda_fwi_amd import psvfwi as P
+    from cuda_fwi_amd import workloads as WL
+    if args.extras:
+        w = WL.c4(nshots=args.nshots)
+        fwi_dt = args.fwi_dt
+        window = w.model_window(dt=fwi_dt)
+        g = _groups_for_store(w, w.nx * w.nz, fwi_dt, 8, args.store_budget_gb * 1e9)[-1]
+        lam, mu, rho = w.padded(*w.background)
+        plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
+                      dz=w.dx, hc=P.holberg(w.fdorder, w.max_relative_error), precision="fp64")
+        plan.set_pml(P.cpml_profiles(w.npml, w.dx, w.fc, 2.0, 1.0, 1e-3, float(np.max(w.vp)), w.dt))
+        plan.set_window(window)
+        plan.set_model(lam, mu, *P.average_parameters(mu, rho))
+        plan.set_shots(_shot_descs(w, g))
+        plan.set_tile(min(abs(line["config"]["tile"]), 16) if line["config"]["tile"] > 0 else 8)
+        plan.set_observed([np.zeros((len(w.shots[i]["rec"]["x"]), w.shots[i]["nt"])) for i in g])
         _sync()
         t0 = time.perf_counter()
-        fwd_adj(p1, [i])
+        plan.forward(energy=False, gradient=True)
+        plan.misfit()
+        plan.adjoint(energy=False, gradient=True)
         _sync()
-        tp += time.perf_counter() - t0
-    del p1
-    per_shot_rate = len(sample) / tp
-
-    cells = w.interior_cells()
-    cell_steps = sum(sh["nt"] for sh in w.shots) * cells
-    kb = alg_bytes(w, nwin, fwi_dt, esize)
-    steps = sum(sh["nt"] for sh in w.shots)
-    alg = steps * sum(kb.values())
-    recompute = args.store != "full"
-    return {
-        "metric": METRIC, "value": round(2.0 * cell_steps / tb / 1e9, 3), "unit": "Gcell-updates/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(tb * 1e3, 1),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if esize == 4 else "f64",
-        "data": "synthetic: seeded layered model with a dipping fault (observed), smoothed (synthetic)",
-        "config": {"workload": f"1500x600 dx5 order-4 npml40, {len(w.shots)} shots, nt U[3000,8000], "
-                               "receivers U[50,400], fwd+adj+grad",
-                   "gradient_store": (f"full decimated store, fwi_dt {fwi_dt}, shots in {len(groups)} "
-                                      f"groups sized to {args.store_budget_gb} GB of store"
-                                      if not recompute else f"checkpoint every {args.interval} steps, "
-                                      "segments recomputed (recomputation not counted)"),
-                   "tile": tile, "shot_order": "shot-major" if band == 0 else f"bands of {band} tile rows",
-                   "launch_group": lgroup or "whole batch", "graph_steps": args.graph_steps},
-        "hbm": {"alg_bytes": int(alg), "alg_GBps": round(alg / tb / 1e9, 1),
-                "frac_of_peak": round(alg / tb / 1e9 / HBM_PEAK_GBS, 4),
-                "note": "algorithmic bytes of forward + adjoint + gradient over the wall time"},
-        "batched_shots_per_s": round(len(w.shots) / tb, 3),
-        "per_shot_shots_per_s": round(per_shot_rate, 3),
-        "per_shot_sample": len(sample),
-        "batched_over_per_shot": round(len(w.shots) / tb / per_shot_rate, 3),
-    }
+        T = time.perf_counter() - t0
+        cells = sum(w.shots[i]["nt"] for i in g) * w.interior_cells()
+        line["fp64"] = {"value": round(2.0 * cells / T / 1e9, 3), "unit": "Gcell-updates/s",
+                        "sample": f"FP64 kernels (the reference's arithmetic), one store group of {len(g)} "
+                                  f"shots, fwd+adj+grad, {T:.2f} s"}
+        del plan
+        _log("fp64 group done")
+        a2 = argparse.Namespace(**vars(args))
+        a2.steps, a2.warmup, a2.nt, a2.tile, a2.no_cpu_baseline = 3, 1, 0, 0, True
+        a2.precision = "fp32"
+        c2 = run_c2(a2)
+        line["c2_single_shot"] = {"value": c2["value"], "unit": c2["unit"], "ms_per_pass": c2["ms_per_step"],
+                                  "workload": c2["config"]["workload"], "roofline_frac": c2["roofline"]["frac"],
+                                  "tile": c2["config"]["tile"]}
+        _log("1000 x 500 single shot done")
+    if not args.no_cpu_baseline:
+        w = WL.c4(nshots=1)
+        window = w.model_window(dt=args.fwi_dt)
+        _log("cpu baseline (oracle) starting")
+        line["cpu_baseline"] = cpu_baseline(w, window, args.cpu_nt)
+        line["cpu_baseline"]["sample"] += "; one C4 shot's grid (1584 x 684 padded), same model and geometry"
+        _log("cpu baseline (threaded oracle) starting")
+        line["cpu_baseline_threads"] = cpu_baseline(w, window, args.cpu_nt, threaded=True)
+    return line
 
 
 def run_c3mem(args):
@@ -603,7 +732,7 @@ def run_c3mem(args):
     free0, total = torch.cuda.mem_get_info()
     mode = "random_boundary" if args.store == "random_boundary" else "checkpoint"
     plan = make_plan(w, args.precision, window, mode, args.interval, args.tile)
-    esize = 4 if args.precision == "fp32" else 8
+    esize = 8 if args.precision in ("fp64", "fp32x2") else 4
     used = []
     _sync()
     t0 = time.perf_counter()
@@ -636,9 +765,9 @@ def run_c3mem(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=7)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c3mem", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c2", "c4", "c3mem", "c5"])
     ap.add_argument("--iters", type=int, default=20, help="c5: FWI iterations")
     ap.add_argument("--c5-group", type=int, default=0,
                     help="c5: shots per batched launch (0: all); the discrete adjoint of all 128 full-size "
@@ -650,24 +779,29 @@ def main():
     ap.add_argument("--c5-save", default="", help="c5: write the model, step and history here after the run")
     ap.add_argument("--c5-scale", type=float, default=0.5,
                     help="c5: lengths, shot and receiver counts and nt relative to 2400 x 800 / 128 / 600 / 6000")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32fma", "fp64", "fp32x2"])
     ap.add_argument("--store", default="full", choices=["full", "checkpoint", "random_boundary"])
     ap.add_argument("--interval", type=int, default=200)
     ap.add_argument("--fwi-dt", type=int, default=5)
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--cpu-nt", type=int, default=400)
+    ap.add_argument("--cpu-nt", type=int, default=200)
+    ap.add_argument("--tune-steps", type=int, default=60, help="c4: steps per autotuner candidate")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="c4: skip the FP64 group and the 1000 x 500 line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--per-shot-sample", type=int, default=8)
+    ap.add_argument("--per-shot-sample", type=int, default=0,
+                    help="c4: shots timed one per launch (0: all of them)")
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="replay the time loop as captured HIP graphs of this many steps (0: direct launches)")
     ap.add_argument("--gradient", default="reference", choices=["reference", "discrete"],
                     help="default workload: the reference's correlation gradient or the discrete (exact) one")
-    ap.add_argument("--launch-group", type=int, default=-1,
-                    help="c4: shots per launch group (-1: the autotuner's choice, 0: whole batch)")
+    ap.add_argument("--launch-group", type=int, default=6,
+                    help="c4: shots per launch group (-1: the autotuner's choice, 0: whole group); 6 was the "
+                         "fastest over whole runs (the autotuner times short runs)")
     ap.add_argument("--store-budget-gb", type=float, default=200.0,
                     help="batched workload with --store full: store memory per group of shots")
     args = ap.parse_args()
@@ -685,6 +819,8 @@ def main():
     if args.workload == "c2" and args.gradient == "discrete":
         run = run_c2_discrete
     line = run(args)
+    if args.workload == "c4":
+        line = c4_extras(args, line)
     print(json.dumps(line), flush=True)
 
 
