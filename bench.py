@@ -583,10 +583,14 @@ def run_c4(args):
     bytes_per_launch = alg_kernel[kname] / launches[kname]
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     alg_total = sum(alg_kernel.values())
+    # measured HBM bytes of this kernel per shot-step (FETCH_SIZE + WRITE_SIZE, copy-calibrated,
+    # from a separate rocprofv3 run), times the shots per launch of this run
     traffic = None
     prof = os.path.join(ROOT, "profiles", "pmc_traffic_c4.json")
-    if os.path.exists(prof):
-        traffic = json.load(open(prof)).get(kname)
+    if os.path.exists(prof) and esize == 4:
+        k = json.load(open(prof))["kernels"].get(kname)
+        if k:
+            traffic = int(k["traffic_bytes_per_shot_step"] * bytes_per_launch / kb[kname])
     batched_rate = shots_done / T
     batched_gcell = 2.0 * cell_steps / T / 1e9
     del plan
@@ -644,9 +648,10 @@ def run_c4(args):
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[kname], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this kernel, "
-                                        "scaled by the copy calibration, from profiles/pmc_traffic_c4.json "
-                                        "(a separate profiled run, not this one)") if traffic else None,
+                     "traffic_source": ("stored profile, not measured in this run: rocprofv3 FETCH_SIZE + "
+                                        "WRITE_SIZE of this kernel per shot-step, copy-calibrated "
+                                        "(profiles/pmc_traffic_c4.json), times this run's shots per launch")
+                                       if traffic else None,
                      "alg_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_us, 2),
                      "launches": launches[kname],
                      "kernel_us_per_launch": {k: round(kus_total[k] / launches[k], 2) for k in kinds}},
