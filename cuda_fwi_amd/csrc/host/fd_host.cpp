@@ -510,6 +510,8 @@ void MI355XOptions::parse(const char* path)
     checkpoint_interval = (int)r.GetInteger(s, "checkpoint_interval", checkpoint_interval);
     tile = (int)r.GetInteger(s, "tile", tile);
     observed = r.Get(s, "observed", observed);
+    model_file = r.Get(s, "model_file", model_file);
+    true_model_file = r.Get(s, "true_model_file", true_model_file);
     gradient_scaling = r.Get(s, "gradient_scaling", gradient_scaling);
     gradient = r.Get(s, "gradient", gradient);
     gradient_segment = (int)r.GetInteger(s, "gradient_segment", gradient_segment);
@@ -520,8 +522,10 @@ void MI355XOptions::parse(const char* path)
         throw std::runtime_error("[mi355x] gradient_store: full or checkpoint");
     if (gradient_scaling != "reference" && gradient_scaling != "clean")
         throw std::runtime_error("[mi355x] gradient_scaling: reference or clean");
-    if (!observed.empty() && observed != "modified" && observed != "file")
-        throw std::runtime_error("[mi355x] observed: empty, modified or file");
+    if (!observed.empty() && observed != "modified" && observed != "file" && observed != "model")
+        throw std::runtime_error("[mi355x] observed: empty, modified, file or model");
+    if (observed == "model" && true_model_file.empty())
+        throw std::runtime_error("[mi355x] observed = model needs true_model_file");
     if (checkpoint_interval < 1) throw std::runtime_error("[mi355x] checkpoint_interval must be >= 1");
     if (gradient != "reference" && gradient != "discrete")
         throw std::runtime_error("[mi355x] gradient: reference or discrete");

@@ -45,6 +45,33 @@ void write_csv(const std::string& path, real_sim** a, int rows, int cols)
 }
 
 // rows of ", "-separated numbers (the format this library and the reference write)
+// vp, vs, rho [nz][nx] (raw FP64, row-major) into lam, mu, rho [nzt][nxt]: the model
+// cells at their padded indices (x + ppad, z + fpad with a free surface, z + ppad without),
+// the padding filled with the nearest model cell; mu = vs^2 rho, lam = vp^2 rho - 2 mu
+// (FDMedium::velocity_to_lame, CUDA_week2/fd_medium.cu:54-59, per cell).
+void load_model(const FDWaveModel& m, const std::string& path, real_sim** lam, real_sim** mu, real_sim** rho)
+{
+    const int nz = m.grid.nz, nx = m.grid.nx, nzt = m.grid.nzt, nxt = m.grid.nxt;
+    const size_t n = (size_t)nz * nx;
+    std::vector<double> v(3 * n);
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open model file " + path);
+    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+    if ((size_t)f.gcount() != v.size() * sizeof(double))
+        throw std::runtime_error("model file " + path + ": expected 3 x nz x nx FP64 values (vp, vs, rho)");
+    const int zoff = m.grid.fsurf ? m.grid.fpad : m.grid.ppad, xoff = m.grid.ppad;
+    for (int iz = 0; iz < nzt; iz++)
+        for (int ix = 0; ix < nxt; ix++) {
+            const int z = std::min(std::max(iz - zoff, 0), nz - 1), x = std::min(std::max(ix - xoff, 0), nx - 1);
+            const size_t k = (size_t)z * nx + x;
+            const double vp = v[k], vs = v[n + k], r = v[2 * n + k];
+            if (!(vp > 0 && vs >= 0 && r > 0)) throw std::runtime_error("model file " + path + ": vp, rho must be > 0");
+            rho[iz][ix] = r;
+            mu[iz][ix] = vs * vs * r;
+            lam[iz][ix] = vp * vp * r - 2.0 * mu[iz][ix];
+        }
+}
+
 void read_csv(const std::string& path, real_sim** a, int rows, int cols)
 {
     std::ifstream in(path);
@@ -331,7 +358,8 @@ void FDWaveModel::forward_simulate()
     const int nzt = grid.nzt, nxt = grid.nxt;
     medium.allocate_medium(nzt, nxt);
     medium.velocity_to_lame();
-    medium.stagger_over_grid_modify(nzt, nxt);
+    if (dev.model_file.empty()) medium.stagger_over_grid_modify(nzt, nxt);
+    else load_model(*this, dev.model_file, medium.lam, medium.mu, medium.rho);
     if (dev.write_csv) write_csv(io_dir(*this, "snap_data") + "/lamda_snap.csv", medium.rho, nzt, nxt);
     checkfd_ssg_elastic(grid.nx, grid.nz, grid.dx, grid.dt, source.fc.empty() ? 0.0 : source.fc[0], pml.npml,
                         medium.rho, medium.lam, medium.mu, hc);
@@ -528,6 +556,14 @@ void FDWaveModel::fwi_simulate()
         download_records(*this, impl_);
         for (unsigned int i = 0; i < receiver.nseis; i++)
             std::memcpy(receiver.signal_meas[i], receiver.signal[i], sizeof(double) * nt);
+    } else if (dev.observed == "model") {
+        load_model(*this, dev.true_model_file, medium.lam, medium.mu, medium.rho);
+        upload_model(*this, p);
+        tune_once(*this, impl_);
+        check(psvfwi_forward(p, 0), "forward");
+        download_records(*this, impl_);
+        for (unsigned int i = 0; i < receiver.nseis; i++)
+            std::memcpy(receiver.signal_meas[i], receiver.signal[i], sizeof(double) * nt);
     } else if (dev.observed == "file") {
         read_csv(receiver.signal_file, receiver.signal_meas, (int)receiver.nseis, nt);
     } else if (!receiver.measurement_records) {
@@ -539,7 +575,8 @@ void FDWaveModel::fwi_simulate()
             for (int i : impl_->rec_idx[s]) obs.insert(obs.end(), receiver.signal_meas[i], receiver.signal_meas[i] + nt);
         check(psvfwi_set_observed(p, obs.data()), "set_observed");
     }
-    medium.stagger_over_grid(nzt, nxt);
+    if (dev.model_file.empty()) medium.stagger_over_grid(nzt, nxt);
+    else load_model(*this, dev.model_file, medium.lam, medium.mu, medium.rho);
     for (int iz = 0; iz < nzt; iz++)
         for (int ix = 0; ix < nxt; ix++) {
             medium.rho_old[iz][ix] = medium.rho[iz][ix];

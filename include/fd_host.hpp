@@ -158,7 +158,12 @@ struct MI355XOptions {
     int checkpoint_interval = 200;
     int tile = 0;                        // 0: autotune
     std::string observed = "";           // "" (reference: zeros), "modified" (forward of the
-                                         // stagger_over_grid_modify model) or "file" (receiver signal_file)
+                                         // stagger_over_grid_modify model), "file" (receiver signal_file)
+                                         // or "model" (forward of true_model_file)
+    std::string model_file = "";         // raw FP64 vp, vs, rho, each [nz][nx] row-major (the unpadded
+                                         // model; edges repeated into the padding): the model in place of
+                                         // the [medium] scalars (the FWI start model)
+    std::string true_model_file = "";    // the same layout: the model observed = model forwards
     std::string gradient_scaling = "reference";  // reference | clean (energy weights, model update)
     std::string gradient = "reference";  // reference (correlation) | discrete (exact misfit gradient;
                                          // implies the clean, window-correct update)

@@ -12,6 +12,7 @@ import pytest
 
 import psv_oracle as O
 import refcase as RC
+from cuda_fwi_amd import psvfwi as P
 from hostbuild import ROOT, build_harness, run
 
 pytestmark = pytest.mark.gpu
@@ -366,3 +367,125 @@ def test_main_fwi_discrete_gradient_converges(tmp_path):
     e1 = np.linalg.norm(lam - pb_true.lam)
     print(f"  lambda error vs the disc model: start {e0:.4e}, after {iters} iterations {e1:.4e}")
     assert e1 < e0
+
+
+MODEL_INI = """\
+[grid]
+nt = {nt}
+nx = {nx}
+nz = {nz}
+dx = 5.0
+dz = 5.0
+dt = 0.5e-3
+free_surface = 0
+
+[medium]
+scalar_rho = 2000.0
+scalar_vp = 2500.0
+scalar_vs = 1443.0
+
+[PML_boundary]
+npml = 10
+fpml = 25.0
+dh = 5.0
+npower = 2.0
+k_max_PML = 1.0
+rcoef = 0.001
+damp_v_PML = 4500.0
+
+[FDM]
+fdorder = 4
+max_relative_error = 1
+full_waveform_inversion = 1
+nshots = {nshots}
+max_iteration = {iters}
+max_error = 1.0e-30
+
+[source]
+nseis = {nshots}
+x_seis = {{{sx}}}
+z_seis = {{{sz}}}
+velocity_or_stress = {{{s0}}}
+component_type = {{{s2}}}
+shot_to_fire = {{{sshot}}}
+shape = {{{s1}}}
+amplitude = {{{samp}}}
+peak_frequency = {{{sfc}}}
+time_shift = {{{s0}}}
+
+[receiver]
+nseis = {nrec}
+x_seis = {{{rx}}}
+z_seis = {{{rz}}}
+velocity_or_stress = {{{r0}}}
+component_type = {{{r2}}}
+shot_to_fire = {{{rshot}}}
+
+[full_waveform_inversion]
+time_interval = 2
+grid_interval_x = 1
+grid_interval_y = 1
+left_grid = 0
+right_grid = {nx}
+top_grid = 0
+bottom_grid = {nz}
+
+[mi355x]
+precision = fp64
+observed = model
+model_file = start.bin
+true_model_file = true.bin
+gradient = discrete
+gradient_segment = 50
+write_csv = false
+"""
+
+
+def test_main_fwi_model_files(tmp_path):
+    """psvfwi_main runs the FWI of the layered / faulted / high-velocity-body configuration
+    (shrunk to 160 x 60 cells, 4 shots x 24 receivers) from model files: observed traces from
+    true_model_file, the start model (the truth smoothed) from model_file.  Its first misfit
+    equals the misfit the C ABI computes for the same padded models and geometry (FP64), and
+    the misfit falls at every iteration (discrete gradient)."""
+    from cuda_fwi_amd import workloads as WL
+    print("psvfwi_main FWI from model files", flush=True)
+    nx, nz, nt, ns, nr = 160, 60, 500, 4, 24
+    w = WL.c5(nshots=ns, nrec=nr, nt=nt, nx=nx, nz=nz)
+    w.npml, w.fc, w.dt = 10, 25.0, 0.5e-3
+    np.concatenate([w.vp.ravel(), w.vs.ravel(), w.rho.ravel()]).astype(np.float64).tofile(tmp_path / "true.bin")
+    vp0, vs0, rho0 = w.background
+    np.concatenate([vp0.ravel(), vs0.ravel(), rho0.ravel()]).astype(np.float64).tofile(tmp_path / "start.bin")
+    sx = [s["src"]["x"][0] for s in w.shots]
+    rx = [x for s in w.shots for x in s["rec"]["x"]]
+    rz = [z + 2 for s in w.shots for z in s["rec"]["z"]]
+    j = lambda v: ", ".join(str(a) for a in v)
+    text = MODEL_INI.format(nt=nt, nx=nx, nz=nz, nshots=ns, iters=3, sx=j(sx), sz=j([2] * ns),
+                            s0=j([0] * ns), s1=j([1] * ns), s2=j([2] * ns), sshot=j(range(ns)),
+                            samp=j([1.0] * ns), sfc=j([25.0] * ns), nrec=len(rx), rx=j(rx), rz=j(rz),
+                            r0=j([0] * len(rx)), r2=j([2] * len(rx)), rshot=j(s for s in range(ns) for _ in range(nr)))
+    out = _main(tmp_path, text)
+    hist = [float(l.split("=")[1].split(",")[0]) for l in out.splitlines() if l.startswith("L2 norm")]
+    print(f"  misfit per iteration {hist}", flush=True)
+    assert len(hist) == 3 and hist[2] < hist[1] < hist[0]
+    # the same misfit through the C ABI: padded models, padded geometry
+    shots = []
+    for i, s in enumerate(w.shots):
+        sig = P.ricker(nt, w.dt, 1.0, 25.0)[None]
+        shots.append(dict(nt=nt, src=dict(x=[sx[i] + w.ppad], z=[2 + w.zoff], comp=[2]), src_signal=sig,
+                          rec=dict(x=[x + w.ppad for x in s["rec"]["x"]], z=[z + 2 + w.zoff for z in s["rec"]["z"]],
+                                   comp=[2] * nr)))
+    plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=4, npml=10, fsurf=0, dt=w.dt, dx=5.0, dz=5.0,
+                  hc=P.holberg(4, 1), precision="fp64")
+    plan.set_pml(P.cpml_profiles(10, 5.0, 25.0, 2.0, 1.0, 1e-3, 4500.0, w.dt))
+    plan.set_shots(shots)
+    lt, mt, rt = w.padded()
+    plan.set_model(lt, mt, *P.average_parameters(mt, rt))
+    plan.forward(energy=False)
+    obs = [plan.records(i) for i in range(ns)]
+    lam, mu, rho = w.padded(*w.background)
+    plan.set_model(lam, mu, *P.average_parameters(mu, rho))
+    plan.forward(energy=False)
+    m0 = float(np.sum(plan.misfit(obs)))
+    plan.close()
+    # psvfwi_main prints the misfit with the stream's default 6 significant digits
+    assert abs(hist[0] - m0) <= 1e-5 * m0, (hist[0], m0)
