@@ -517,7 +517,8 @@ void MI355XOptions::parse(const char* path)
     gradient_segment = (int)r.GetInteger(s, "gradient_segment", gradient_segment);
     output_dir = r.Get(s, "output_dir", output_dir);
     write_csv = r.GetBoolean(s, "write_csv", write_csv);
-    if (precision != "fp32" && precision != "fp64") throw std::runtime_error("[mi355x] precision: fp32 or fp64");
+    if (precision != "fp32" && precision != "fp64" && precision != "fp32x2" && precision != "fp32fma")
+        throw std::runtime_error("[mi355x] precision: fp32, fp64, fp32x2 or fp32fma");
     if (gradient_store != "full" && gradient_store != "checkpoint")
         throw std::runtime_error("[mi355x] gradient_store: full or checkpoint");
     if (gradient_scaling != "reference" && gradient_scaling != "clean")

@@ -255,7 +255,11 @@ static void build_plan(FDWaveModel& m, FDWaveModel::Impl* impl, bool with_window
     if (impl->plan) psvfwi_plan_destroy(impl->plan);
     impl->plan = nullptr;
     impl->tuned = false;
-    check(psvfwi_plan_create(&g, m.dev.precision == "fp64" ? PSVFWI_FP64 : PSVFWI_FP32, &impl->plan), "plan_create");
+    const psvfwi_precision prec = m.dev.precision == "fp64"     ? PSVFWI_FP64
+                                  : m.dev.precision == "fp32x2" ? PSVFWI_FP32X2
+                                  : m.dev.precision == "fp32fma" ? PSVFWI_FP32_FMA
+                                                                 : PSVFWI_FP32;
+    check(psvfwi_plan_create(&g, prec, &impl->plan), "plan_create");
     psvfwi_plan* p = impl->plan;
     if (m.pml.npml > 0) {
         const int tp = 2 * (m.pml.npml + 1);

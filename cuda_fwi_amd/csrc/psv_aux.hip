@@ -172,5 +172,11 @@ template hipError_t launch_misfit<dfloat>(const dfloat*, const double*, dfloat*,
 template hipError_t launch_grad_sum<dfloat>(const dfloat*, size_t, const int*, int, size_t, double*,
                                             hipStream_t);
 template hipError_t launch_fill<dfloat>(dfloat*, size_t, dfloat, hipStream_t);
+template hipError_t launch_misfit<ffloat>(const ffloat*, const double*, ffloat*, const size_t*,
+                                          const size_t*, int, int, double, double*, double*, int*,
+                                          hipStream_t);
+template hipError_t launch_grad_sum<ffloat>(const ffloat*, size_t, const int*, int, size_t, double*,
+                                            hipStream_t);
+template hipError_t launch_fill<ffloat>(ffloat*, size_t, ffloat, hipStream_t);
 
 }  // namespace psv

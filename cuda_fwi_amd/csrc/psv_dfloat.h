@@ -126,3 +126,33 @@ __host__ __device__ inline dfloat operator/(dfloat x, dfloat y)
 }
 
 }  // namespace psv
+
+namespace psv {
+
+// FP32 whose multiplies and adds may fuse into FMA instructions: the library is
+// built with contraction off (the FP32 and FP64 kernels round every operation as
+// the CPU path does), and this type's operators carry the contraction permission
+// themselves, so the kernels instantiated with it let the compiler turn a * b + c
+// into one FMA.  Results are FP32 results with fewer roundings, not bit-identical
+// to the plain FP32 kernels.  Precision PSVFWI_FP32_FMA.
+struct __align__(4) ffloat {
+    float v;
+
+    ffloat() = default;
+    __host__ __device__ constexpr ffloat(float f) : v(f) {}
+    __host__ __device__ constexpr ffloat(double d) : v((float)d) {}
+    __host__ __device__ constexpr ffloat(int i) : v((float)i) {}
+    __host__ __device__ explicit operator double() const { return (double)v; }
+};
+
+#pragma clang fp contract(fast)
+__host__ __device__ inline ffloat operator+(ffloat a, ffloat b) { return ffloat(a.v + b.v); }
+__host__ __device__ inline ffloat operator-(ffloat a, ffloat b) { return ffloat(a.v - b.v); }
+__host__ __device__ inline ffloat operator*(ffloat a, ffloat b) { return ffloat(a.v * b.v); }
+__host__ __device__ inline ffloat operator/(ffloat a, ffloat b) { return ffloat(a.v / b.v); }
+__host__ __device__ inline ffloat operator-(ffloat a) { return ffloat(-a.v); }
+#pragma clang fp contract(off)
+__host__ __device__ inline bool operator==(ffloat a, ffloat b) { return a.v == b.v; }
+__host__ __device__ inline bool operator!=(ffloat a, ffloat b) { return a.v != b.v; }
+
+}  // namespace psv

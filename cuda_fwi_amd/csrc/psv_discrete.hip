@@ -449,5 +449,6 @@ hipError_t launch_discrete_step(const DiscreteParams<T>& p, int order, hipStream
 template hipError_t launch_discrete_step<float>(const DiscreteParams<float>&, int, hipStream_t);
 template hipError_t launch_discrete_step<double>(const DiscreteParams<double>&, int, hipStream_t);
 template hipError_t launch_discrete_step<dfloat>(const DiscreteParams<dfloat>&, int, hipStream_t);
+template hipError_t launch_discrete_step<ffloat>(const DiscreteParams<ffloat>&, int, hipStream_t);
 
 }  // namespace psv

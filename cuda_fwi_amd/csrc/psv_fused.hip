@@ -575,5 +575,6 @@ hipError_t launch_fused(const StepParams<T>& p, int order, int mode, int tz, hip
 template hipError_t launch_fused<float>(const StepParams<float>&, int, int, int, hipStream_t);
 template hipError_t launch_fused<double>(const StepParams<double>&, int, int, int, hipStream_t);
 template hipError_t launch_fused<dfloat>(const StepParams<dfloat>&, int, int, int, hipStream_t);
+template hipError_t launch_fused<ffloat>(const StepParams<ffloat>&, int, int, int, hipStream_t);
 
 }  // namespace psv

@@ -64,43 +64,64 @@ struct Pml {
 
 }  // namespace
 
-template <typename T, int ORDER, int MODE, int S>
-__global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T> p)
+constexpr int kMarchR = 4;     // rows per iteration, one per wave
+constexpr int kMarchVB = 16;   // rows of the velocity ring (power of two)
+constexpr int kMarchSB = 8;    // rows of the stress ring (power of two)
+constexpr int kMarchW = kTX + 4;  // ring columns x0-2 .. x0+65
+
+template <typename T, int S>
+struct MarchLds {
+    T lvx[kMarchVB][kMarchW];
+    T lvz[kMarchVB][kMarchW];
+    T lsxx[kMarchSB][kMarchW];
+    T lszx[kMarchSB][kMarchW];
+    T lszz[kMarchSB][kTX];
+    T lring[S][6];  // sxx at x0-1, x0+64, x0+65; szx at x0-2, x0-1, x0+64
+};
+
+// A segment is interior when every stress cell it computes (rows zs-2 .. ze+1, columns
+// x0-2 .. x0+65) is an updated cell outside every CPML strip and below the free-surface
+// row: then none of the boundary code runs (EDGE = false below).
+template <int ORDER>
+__device__ __forceinline__ bool march_interior(int zs, int ze, int x0, int nzt, int nxt, int npml, int ppad,
+                                               bool fsurf)
+{
+    constexpr int H = ORDER / 2;
+    const int zlo = fsurf ? H + 1 : (npml > 0 ? ppad + 1 : H);
+    const int zhi = npml > 0 ? nzt - ppad - 1 : nzt - H;
+    const int xlo = npml > 0 ? ppad + 1 : H;
+    const int xhi = npml > 0 ? nxt - ppad - 1 : nxt - H;
+    return zs - 2 >= zlo && ze + 2 <= zhi && x0 - 2 >= xlo && x0 + kTX + 2 <= xhi;
+}
+
+template <typename T, int ORDER, int MODE, int S, bool EDGE>
+__device__ __forceinline__ void march_segment(const StepParams<T>& p, MarchLds<T, S>& L, int slot, int zs,
+                                              int x0, int it)
 {
     constexpr int H = ORDER / 2;       // fpad
-    constexpr int R = 4;               // rows per iteration, one per wave
-    constexpr int VB = 16;             // rows of the velocity ring (power of two)
-    constexpr int SB = 8;              // rows of the stress ring (power of two)
-    constexpr int W = kTX + 4;         // ring columns x0-2 .. x0+65
+    constexpr int R = kMarchR;
+    constexpr int VB = kMarchVB;
+    constexpr int SB = kMarchSB;
     constexpr int NRING = 6 * S;       // prologue cells: 3 sxx and 3 szx columns per row
     static_assert(S % R == 0, "segment rows");
-    __shared__ T lvx[VB][W];
-    __shared__ T lvz[VB][W];
-    __shared__ T lsxx[SB][W];
-    __shared__ T lszx[SB][W];
-    __shared__ T lszz[SB][kTX];
-    __shared__ T lring[S][6];          // sxx at x0-1, x0+64, x0+65; szx at x0-2, x0-1, x0+64
-
-    const int ntile = p.tiles_x * p.tiles_z;
-    const int bid = xcd_spread(blockIdx.x, ntile * p.nslots);
-    const int slot = bid / ntile, tile = bid - slot * ntile;
-    const int tzi = tile / p.tiles_x;
-    const int zs = tzi * S, x0 = (tile - tzi * p.tiles_x) * kTX;
-    const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;
-    const int nt_s = p.shot_nt[slot];
-    if (it >= nt_s) return;  // this shot has finished (uniform over the block)
+    auto& lvx = L.lvx;
+    auto& lvz = L.lvz;
+    auto& lsxx = L.lsxx;
+    auto& lszx = L.lszx;
+    auto& lszz = L.lszz;
+    auto& lring = L.lring;
 
     const int nzt = p.nzt, nxt = p.nxt, pitch = p.pitch, org = p.org;
     const int ze = min(zs + S, nzt);
     const int fpad = H, ppad = p.ppad;
     const int tid = threadIdx.x;
     const int tx = tid & (kTX - 1), wv = tid / kTX;
-    const bool fsurf = p.fsurf != 0;
+    const bool fsurf = EDGE && p.fsurf != 0;
     const bool energy = (p.flags & kEnergy) != 0;
     const bool fwi_step = (p.flags & kStore) && (it % p.fwi_dt == 0);
     const bool unit_stride = p.fwi_dz == 1 && p.fwi_dx == 1;
     const int tf_cur = it / p.fwi_dt - p.tf0;
-    const int npml = p.npml, tp = p.tp;
+    const int npml = EDGE ? p.npml : 0, tp = p.tp;
     const T dt = p.dt, dxi = p.dxi, dzi = p.dzi, hc1 = p.hc1, hc2 = p.hc2;
     const Pml<T> pm{p.pml, p.pml + tp, p.pml + 2 * tp, p.pml + 3 * tp, p.pml + 4 * tp, p.pml + 5 * tp};
     const size_t xo = (size_t)slot * p.xst, zo = (size_t)slot * p.zst;
@@ -127,7 +148,9 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
     const T* rxpM = p.rho_xp - org;
     const T* rzpM = p.rho_zp - org;
     auto boff = [&](int gz, int gx) -> unsigned { return (unsigned)((org + gz * pitch + gx) * (int)sizeof(T)); };
-    auto interior = [&](int gz, int gx) { return gz >= fpad && gz < nzt - fpad && gx >= fpad && gx < nxt - fpad; };
+    auto interior = [&](int gz, int gx) {
+        return !EDGE || (gz >= fpad && gz < nzt - fpad && gx >= fpad && gx < nxt - fpad);
+    };
 
     // ---- the CPML of the stress half-step: x strips (left ix <= ppad, then right) and z
     // strips (top iz <= ppad without a free surface, then bottom), each applied in turn as
@@ -294,11 +317,12 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
             lvz[rs][lc] = ldm(vzA, oh);
         }
     }
-    Ops nx;
-    load_ops(0, nx);
-
-    for (int i = 0; i < niter; i++) {
-        const Ops cu = nx;
+    // operands are loaded two iterations ahead (two register sets in flight), so a
+    // workgroup waits on memory once per two iterations at most
+    Ops qa, qb;
+    load_ops(0, qa);
+    if (niter > 1) load_ops(1, qb);
+    auto iteration = [&](int i, const Ops& cu) {
         const int zr = zs - 2 + R * i;
         {
             const int rs = (zr + 2 + wv) & (VB - 1);
@@ -310,7 +334,6 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
                 lvz[rs][lc] = cu.vzh;
             }
         }
-        if (i + 1 < niter) load_ops(i + 1, nx);
         __syncthreads();
 
         // ---- stress row gz = zr + wv
@@ -318,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
             const int gz = zr + wv, gx = x0 + tx;
             const int vr = gz & (VB - 1), vc = 2 + tx;
             const bool own = gz >= zs && gz < ze;
-            const bool ingrid = gz >= 0 && gz < nzt && gx < nxt;
+            const bool ingrid = !EDGE || (gz >= 0 && gz < nzt && gx < nxt);
             // ghost row above the free surface: its szz and szx come from the mirror below
             const bool mcol = gx >= fpad && gx < nxt - fpad;
             const bool ghost = fsurf && gz >= 0 && gz < fpad && mcol;
@@ -423,7 +446,7 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
             // about row fpad at order 2 and about fpad - 1/2 at order 4
             T gszz = 0, gszx = 0;
             int ggz = -1, ggx = -1;
-            if (fsurf && zs == 0 && mcol) {
+            if (EDGE && fsurf && zs == 0 && mcol) {
                 if (gz > fpad && gz <= 2 * fpad) {
                     ggz = 2 * fpad - gz;
                     lszz[ggz & (SB - 1)][tx] = -szzn;
@@ -453,7 +476,7 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
         // ---- velocity row gz = zr - 2 + wv
         {
             const int gz = zr - 2 + wv, gx = x0 + tx;
-            if (gz >= zs && gz < ze && gx < nxt) {
+            if (gz >= zs && gz < ze && (!EDGE || gx < nxt)) {
                 const int vr = gz & (VB - 1), vc = 2 + tx;
                 const int sr = gz & (SB - 1), sc = 2 + tx;
                 T vxn = lvx[vr][vc], vzn = lvz[vr][vc];
@@ -508,6 +531,40 @@ __global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T>
                 stm(vzB, o, vzn);
             }
         }
+    };
+    for (int i = 0; i < niter; i += 2) {
+        {
+            const Ops cu = qa;
+            if (i + 2 < niter) load_ops(i + 2, qa);
+            iteration(i, cu);
+        }
+        if (i + 1 < niter) {
+            const Ops cu = qb;
+            if (i + 3 < niter) load_ops(i + 3, qb);
+            iteration(i + 1, cu);
+        }
+    }
+}
+
+// PATH 0: every segment of the launch, interior ones on the lean path; 1: interior segments
+// only; 2: the other segments only.
+template <typename T, int ORDER, int MODE, int S, int PATH>
+__global__ __launch_bounds__(kThreads) void psv_march_kernel(const StepParams<T> p)
+{
+    __shared__ MarchLds<T, S> lds;
+    const int ntile = p.tiles_x * p.tiles_z;
+    const int bid = xcd_spread(blockIdx.x, ntile * p.nslots);
+    const int slot = bid / ntile, tile = bid - slot * ntile;
+    const int tzi = tile / p.tiles_x;
+    const int zs = tzi * S, x0 = (tile - tzi * p.tiles_x) * kTX;
+    const int it = p.it_dev ? (*p.it_dev + p.it) : p.it;
+    if (it >= p.shot_nt[slot]) return;  // this shot has finished (uniform over the block)
+    const int ze = min(zs + S, p.nzt);
+    const bool inner = march_interior<ORDER>(zs, ze, x0, p.nzt, p.nxt, p.npml, p.ppad, p.fsurf != 0);
+    if (inner) {
+        if (PATH != 2) march_segment<T, ORDER, MODE, S, false>(p, lds, slot, zs, x0, it);
+    } else {
+        if (PATH != 1) march_segment<T, ORDER, MODE, S, true>(p, lds, slot, zs, x0, it);
     }
 }
 
@@ -550,7 +607,10 @@ template <typename T, int ORDER, int MODE, int S>
 static hipError_t launch_march_one(const StepParams<T>& p, hipStream_t s)
 {
     const int blocks = p.tiles_x * p.tiles_z * p.nslots;
-    hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S>), dim3(blocks), dim3(kThreads), 0, s, p);
+    // interior segments on the lean instance (no boundary code: a third of the registers,
+    // twice the waves per SIMD), then the segments along the grid's edges
+    hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 1>), dim3(blocks), dim3(kThreads), 0, s, p);
+    hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 2>), dim3(blocks), dim3(kThreads), 0, s, p);
     hipLaunchKernelGGL((psv_point_kernel<T>), dim3(p.nslots), dim3(256), 0, s, p);
     if (p.nexpl > 0) {
         StepParams<T> q = p;
@@ -579,5 +639,6 @@ hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, h
 template hipError_t launch_march<float>(const StepParams<float>&, int, int, int, hipStream_t);
 template hipError_t launch_march<double>(const StepParams<double>&, int, int, int, hipStream_t);
 template hipError_t launch_march<dfloat>(const StepParams<dfloat>&, int, int, int, hipStream_t);
+template hipError_t launch_march<ffloat>(const StepParams<ffloat>&, int, int, int, hipStream_t);
 
 }  // namespace psv

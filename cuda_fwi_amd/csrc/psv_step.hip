@@ -615,6 +615,7 @@ hipError_t launch_stress_sources(const StepParams<T>& p, hipStream_t s)
 template hipError_t launch_stress_sources<float>(const StepParams<float>&, hipStream_t);
 template hipError_t launch_stress_sources<double>(const StepParams<double>&, hipStream_t);
 template hipError_t launch_stress_sources<dfloat>(const StepParams<dfloat>&, hipStream_t);
+template hipError_t launch_stress_sources<ffloat>(const StepParams<ffloat>&, hipStream_t);
 
 template <typename T, int ORDER, int MODE, int TZ>
 static hipError_t launch_one(const StepParams<T>& p, hipStream_t s, hipEvent_t e_mid)
@@ -676,6 +677,7 @@ hipError_t launch_store_plane(const StepParams<T>& p, hipStream_t s)
 template hipError_t launch_store_plane<float>(const StepParams<float>&, hipStream_t);
 template hipError_t launch_store_plane<double>(const StepParams<double>&, hipStream_t);
 template hipError_t launch_store_plane<dfloat>(const StepParams<dfloat>&, hipStream_t);
+template hipError_t launch_store_plane<ffloat>(const StepParams<ffloat>&, hipStream_t);
 
 #define PSV_HALF(O, M, Z) \
     if (order == O && mode == M && tz == Z) return launch_half_one<T, O, M, Z>(p, which, s);
@@ -698,6 +700,7 @@ hipError_t launch_half(const StepParams<T>& p, int order, int mode, int tz, int 
 template hipError_t launch_half<float>(const StepParams<float>&, int, int, int, int, hipStream_t);
 template hipError_t launch_half<double>(const StepParams<double>&, int, int, int, int, hipStream_t);
 template hipError_t launch_half<dfloat>(const StepParams<dfloat>&, int, int, int, int, hipStream_t);
+template hipError_t launch_half<ffloat>(const StepParams<ffloat>&, int, int, int, int, hipStream_t);
 
 #define PSV_CASE(O, M, Z) \
     if (order == O && mode == M && tz == Z) return launch_one<T, O, M, Z>(p, s, e_mid);
@@ -721,5 +724,6 @@ hipError_t launch_step(const StepParams<T>& p, int order, int mode, int tz, hipS
 template hipError_t launch_step<float>(const StepParams<float>&, int, int, int, hipStream_t, hipEvent_t);
 template hipError_t launch_step<double>(const StepParams<double>&, int, int, int, hipStream_t, hipEvent_t);
 template hipError_t launch_step<dfloat>(const StepParams<dfloat>&, int, int, int, hipStream_t, hipEvent_t);
+template hipError_t launch_step<ffloat>(const StepParams<ffloat>&, int, int, int, hipStream_t, hipEvent_t);
 
 }  // namespace psv

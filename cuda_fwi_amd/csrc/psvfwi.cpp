@@ -2046,19 +2046,23 @@ psvfwi_status psvfwi_plan_create(const psvfwi_grid_desc* g, psvfwi_precision pre
     if (g->npml > 0 && (g->nxt < 2 * ppad || g->nzt < (g->fsurf ? ppad + fpad : 2 * ppad)))
         return fail(PSVFWI_E_ARG, "plan_create: CPML wider than the grid");
     if (!(g->dt > 0) || !(g->dx > 0) || !(g->dz > 0)) return fail(PSVFWI_E_ARG, "plan_create: dt, dx, dz must be > 0");
-    if (prec != PSVFWI_FP32 && prec != PSVFWI_FP64 && prec != PSVFWI_FP32X2)
+    if (prec != PSVFWI_FP32 && prec != PSVFWI_FP64 && prec != PSVFWI_FP32X2 && prec != PSVFWI_FP32_FMA)
         return fail(PSVFWI_E_ARG, "plan_create: unknown precision");
     {
         // the kernels address a shot's full-grid arrays with 32-bit byte offsets
         const long long tiles_x = (g->nxt + psv::kTX - 1) / psv::kTX;
         const long long pitch = (psv::kGuardLeft + tiles_x * psv::kTX + psv::kGuardRight + 31) / 32 * 32;
         const long long rows = 2 * psv::kGuardRows + ((long long)g->nzt + psv::kMaxTz - 1) / psv::kMaxTz * psv::kMaxTz;
-        const long long bytes = rows * pitch * (prec == PSVFWI_FP32 ? 4 : 8);
+        const long long bytes = rows * pitch * (prec == PSVFWI_FP32 || prec == PSVFWI_FP32_FMA ? 4 : 8);
         if (bytes >= (1LL << 31)) return fail(PSVFWI_E_ARG, "plan_create: grid too large (a field of one shot must stay below 2 GiB)");
     }
     std::unique_ptr<psvfwi_plan> p;
     if (prec == PSVFWI_FP32) {
         auto q = std::make_unique<Plan<float>>(*g);
+        PSV_TRY(q->init());
+        p = std::move(q);
+    } else if (prec == PSVFWI_FP32_FMA) {
+        auto q = std::make_unique<Plan<psv::ffloat>>(*g);
         PSV_TRY(q->init());
         p = std::move(q);
     } else if (prec == PSVFWI_FP32X2) {

@@ -20,8 +20,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpsvfwi.so")
 
-FP32, FP64, FP32X2 = 0, 1, 2
-PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32x2": FP32X2}
+FP32, FP64, FP32X2, FP32_FMA = 0, 1, 2, 3
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp32x2": FP32X2, "fp32fma": FP32_FMA}
 ENERGY, GRADIENT = 1, 2
 STORE_FULL, STORE_CHECKPOINT = 0, 1
 FIELDS = {"vx": 0, "vz": 1, "sxx": 2, "szx": 3, "szz": 4, "We": 5, "ux": 6, "uz": 7,

@@ -153,7 +153,7 @@ void checkfd_ssg_elastic(int NX, int NY, real_sim DH, real_sim DT, real_sim TS, 
 
 // device options of the [mi355x] INI section (all optional)
 struct MI355XOptions {
-    std::string precision = "fp32";     // fp32 | fp64
+    std::string precision = "fp32";     // fp32 | fp64 | fp32x2 (compensated FP32) | fp32fma
     std::string gradient_store = "full"; // full | checkpoint
     int checkpoint_interval = 200;
     int tile = 0;                        // 0: autotune

@@ -39,13 +39,21 @@ typedef enum psvfwi_status {
     PSVFWI_E_NOMEM = -6       /* device allocation failed */
 } psvfwi_status;
 
-/* Arithmetic of a plan's kernels.  PSVFWI_FP32X2 (no counterpart in the reference, which
- * computes in FP64, CUDA_week2/globvar.cuh:16) is compensated FP32: every value is an
- * unevaluated sum of two floats and every operation a double-word algorithm built from FP32
- * operations (about 48 significant bits, 8 bytes per value).  It is for cases whose
+/* Arithmetic of a plan's kernels (the reference computes in FP64,
+ * CUDA_week2/globvar.cuh:16).  PSVFWI_FP32 and PSVFWI_FP64 round every operation on its
+ * own, as the CPU path does (FP64 equals it bit for bit).  PSVFWI_FP32_FMA is the throughput
+ * precision: FP32 whose multiply-adds fuse into FMA instructions (fewer instructions and
+ * roundings; not bit-identical to PSVFWI_FP32).  PSVFWI_FP32X2 is compensated FP32: every
+ * value is an unevaluated sum of two floats and every operation a double-word algorithm
+ * built from FP32 operations (about 48 significant bits, 8 bytes per value), for cases whose
  * residual is far below the wavefield (a small anomaly), where plain FP32 rounds the
  * difference of two forward runs away. */
-typedef enum psvfwi_precision { PSVFWI_FP32 = 0, PSVFWI_FP64 = 1, PSVFWI_FP32X2 = 2 } psvfwi_precision;
+typedef enum psvfwi_precision {
+    PSVFWI_FP32 = 0,
+    PSVFWI_FP64 = 1,
+    PSVFWI_FP32X2 = 2,
+    PSVFWI_FP32_FMA = 3
+} psvfwi_precision;
 
 /* flags of psvfwi_forward / psvfwi_adjoint */
 #define PSVFWI_ENERGY 1   /* accumulate the energy weight We (fd_wave.cuh:14-29) */
