@@ -34,6 +34,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "psv_aux.h"
 #include "psv_discrete.h"
 #include "psv_step.h"
@@ -42,6 +44,15 @@
 namespace {
 
 thread_local std::string g_err;
+
+// A ROCTX range around a C ABI call (forward, misfit, adjoint, autotune), seen by
+// rocprofv3 --marker-trace beside the kernels it launched
+struct Range {
+    explicit Range(const char* what) { roctxRangePush(what); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 psvfwi_status fail(psvfwi_status st, const std::string& msg)
 {
@@ -2127,6 +2138,7 @@ psvfwi_status psvfwi_set_tile(psvfwi_plan* plan, int tz)
 psvfwi_status psvfwi_autotune(psvfwi_plan* plan, int steps, int* best_tile)
 {
     PSV_PLAN(plan);
+    const Range r("psvfwi_autotune");
     return plan->autotune(steps, best_tile);
 }
 
@@ -2181,6 +2193,7 @@ psvfwi_status psvfwi_get_snapshot(const psvfwi_plan* plan, int shot, int k, doub
 psvfwi_status psvfwi_forward(psvfwi_plan* plan, int flags)
 {
     PSV_PLAN(plan);
+    const Range r("psvfwi_forward");
     return plan->forward(flags);
 }
 
@@ -2193,12 +2206,14 @@ psvfwi_status psvfwi_set_observed(psvfwi_plan* plan, const double* observed)
 psvfwi_status psvfwi_misfit(psvfwi_plan* plan, const double* observed, double* misfit_out)
 {
     PSV_PLAN(plan);
+    const Range r("psvfwi_misfit");
     return plan->misfit(observed, misfit_out);
 }
 
 psvfwi_status psvfwi_misfit_noalloc(psvfwi_plan* plan, double* misfit_out)
 {
     PSV_PLAN(plan);
+    const Range r("psvfwi_misfit_noalloc");
     return plan->misfit_noalloc(misfit_out);
 }
 
@@ -2217,6 +2232,7 @@ psvfwi_status psvfwi_set_store(psvfwi_plan* plan, int shot, const double* store)
 psvfwi_status psvfwi_adjoint(psvfwi_plan* plan, int flags)
 {
     PSV_PLAN(plan);
+    const Range r((flags & PSVFWI_GRADIENT) ? "psvfwi_adjoint_gradient" : "psvfwi_adjoint");
     return plan->adjoint(flags);
 }
 

@@ -140,6 +140,28 @@ def test_gradient_fp32_bit_exact_and_tolerance(fdorder):
     assert max(eg) < 1e-4
 
 
+@pytest.mark.parametrize("tz", [8, -16, -64])
+@pytest.mark.parametrize("prec", ["fp32fma", "fp32x2"])
+@pytest.mark.parametrize("fdorder", [2, 4])
+def test_gradient_other_precisions_within_tolerance(fdorder, prec, tz):
+    """FP32 with contracted multiply-adds and compensated FP32, on the split, fused-tile and
+    z-marching steps, against the FP64 oracle at the stated tolerances (seismograms 1e-5,
+    gradients 1e-4); compensated FP32 within 1e-9 of it."""
+    print(f"gradient {prec} order {fdorder} tile {tz}", flush=True)
+    pb, obs = _fwi_problem(fdorder=fdorder, nt=400)
+    f64, m64, res64, a64 = _oracle_gradient(pb, obs)
+    plan = plan_of(pb, prec, tz=tz)
+    plan.forward(energy=True, gradient=True)
+    mis = plan.misfit([obs])[0]
+    plan.adjoint(energy=True, gradient=True)
+    es = rel_l2(plan.records(), f64["rec"])
+    eg = [rel_l2(got, a64[k]) for got, k in zip(plan.gradient(0), ("grad_lam", "grad_mu", "grad_rho"))]
+    print(f"  {prec} vs fp64: seismogram {es:.2e}, gradients {eg}, misfit {abs(mis - m64) / m64:.2e}")
+    assert es < 1e-5 and max(eg) < 1e-4
+    if prec == "fp32x2":
+        assert es < 1e-9 and max(eg) < 1e-9
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_checkpoint_gradient_equals_full_store(prec):
     print(f"checkpoint vs store {prec}", flush=True)

@@ -90,9 +90,11 @@ bottom_grid = 0;
 """
 
 
-@pytest.fixture(scope="module")
-def harness(tmp_path_factory):
-    return build_harness(tmp_path_factory.mktemp("hh"))
+@pytest.fixture(scope="module", params=["plain", "asan+ubsan"])
+def harness(tmp_path_factory, request):
+    """The harness against the library, and (asan+ubsan) with the host layer's sources
+    compiled into it under AddressSanitizer and UndefinedBehaviorSanitizer."""
+    return build_harness(tmp_path_factory.mktemp("hh"), sanitize=request.param == "asan+ubsan")
 
 
 def _ini(harness, tmp_path, text, keys, bom=False):
