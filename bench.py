@@ -678,7 +678,7 @@ def c4_extras(args, line):
         w = WL.c4(nshots=args.nshots)
         fwi_dt = args.fwi_dt
         window = w.model_window(dt=fwi_dt)
-        g = _groups_for_store(w, w.nx * w.nz, fwi_dt, 8, args.store_budget_gb * 1e9)[-1]
+        g = max(_groups_for_store(w, w.nx * w.nz, fwi_dt, 8, args.store_budget_gb * 1e9), key=len)
         lam, mu, rho = w.padded(*w.background)
         plan = P.Plan(nzt=w.nzt, nxt=w.nxt, fdorder=w.fdorder, npml=w.npml, fsurf=w.fsurf, dt=w.dt, dx=w.dx,
                       dz=w.dx, hc=P.holberg(w.fdorder, w.max_relative_error), precision="fp64")
