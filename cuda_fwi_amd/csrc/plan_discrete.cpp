@@ -221,7 +221,6 @@ psv::DiscreteParams<T> Plan<T>::dparams(int it, const View& pre, const View& pos
     return d;
 }
 
-// ------------------------------------------------------------------ getters
 template <typename T>
 psvfwi_status Plan<T>::get_coefficient_gradient(int shot, double* gl, double* gm, double* gmzx, double* grzp, double* grxp) const
 {

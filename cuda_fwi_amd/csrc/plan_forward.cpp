@@ -4,7 +4,6 @@
 
 namespace psvplan {
 
-// ------------------------------------------------------------------ runs
 template <typename T>
 psvfwi_status Plan<T>::forward(int flags)
 {
@@ -29,7 +28,7 @@ psvfwi_status Plan<T>::forward(int flags)
     const int kint = discrete ? disc_k_ : ck_k_;
     // the store is not cleared: the adjoint reads exactly the entries the
     // forward writes, and get_store zeroes the ones the window loops never
-    // write (see window_written)
+    // write
     if (full) PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
     if (ckpt) PSV_TRY(alloc_checkpoints(kint));
     const bool snap = snap_interval_ > 0;

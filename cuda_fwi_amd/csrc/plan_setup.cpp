@@ -47,7 +47,6 @@ psvfwi_status Plan<T>::init()
     return PSVFWI_OK;
 }
 
-// ------------------------------------------------------------------ setup
 template <typename T>
 psvfwi_status Plan<T>::set_pml(const double* prof)
 {
