@@ -377,8 +377,11 @@ def run_c5(args):
     start = w.padded(*w.background)
     win = (w.zoff, w.zoff + nz, w.ppad, w.ppad + nx)
     inv = Inversion(plan, shots, observed, start, win, segment=args.interval, mute=args.c5_mute,
-                    group=args.c5_group or None)
+                    group=args.c5_group or None, smooth=args.c5_smooth * sc, precond=args.c5_precond)
     err0 = inv.model_error(truth)
+    # vp errors after low-passing both models: 50 m and 150 m (the start model's smoothing) at full scale
+    sigmas = (10.0 * sc, 30.0 * sc)
+    low0 = [inv.model_error(truth, sig) for sig in sigmas]
     # a run longer than one job continues from the state a previous job saved:
     # the FP64 model over the window (the padding is edge replication of it),
     # the last step length and the per-iteration history
@@ -417,10 +420,16 @@ def run_c5(args):
                 "truth smoothed over 150 m (scaled)",
         "config": {"workload": f"FWI {nx}x{nz} dx5 order-4 npml40, {nshots} shots x {nrec} receivers, nt {nt}, "
                                f"8 Hz, {args.iters} iterations, discrete gradient, checkpoints every "
-                               f"{args.interval} steps, top {args.c5_mute} model rows not updated",
+                               f"{args.interval} steps, top {args.c5_mute} model rows not updated"
+                               + (f", direction smoothed over {args.c5_smooth:g} cells (full scale)"
+                                  if args.c5_smooth > 0 else "")
+                               + (", preconditioner sqrt(Ws Wr)" if args.c5_precond == "source_receiver" else ""),
                    "scale_of_2400x800_128_shots": sc, "tile": plan.tile,
                    "launch_group": plan.launch_group() or "whole batch"},
         "fwi": {"misfit": prev["misfit"], "vp_rel_error_start": err0, "vp_rel_error": prev["model_error"],
+                "vp_rel_error_lowpass": {f"sigma {sig:g} cells": {"start": a, "end": inv.model_error(truth, sig)}
+                                         for sig, a in zip(sigmas, low0)},
+                "direction_peak": inv.diag,
                 "step": prev["step"], "trial_forwards": prev["trials"],
                 "seconds": [round(x, 2) for x in prev["seconds"]], "iterations_this_run": args.iters,
                 "note": "value counts forward + adjoint cell updates of each iteration; trial forwards "
@@ -779,6 +788,10 @@ def main():
                          "shots does not fit in HBM at once")
     ap.add_argument("--c5-mute", type=int, default=10,
                     help="c5: model rows below the surface with no update (sources and receivers at row 2)")
+    ap.add_argument("--c5-smooth", type=float, default=0.0,
+                    help="c5: Gaussian smoothing of the descent direction, sigma in cells at full scale (0: none)")
+    ap.add_argument("--c5-precond", default="source", choices=["source", "source_receiver"],
+                    help="c5: illumination that divides the gradient: source energy, or sqrt(source x receiver)")
     ap.add_argument("--c5-resume", default="",
                     help="c5: continue from a state file written by --c5-save")
     ap.add_argument("--c5-save", default="", help="c5: write the model, step and history here after the run")

@@ -8,9 +8,11 @@ with the batched, window-correct choices of the C++ driver
 - per iteration one batched forward with the energy weights, the FP64 misfit
   sum_shots sum_rec sum_t 0.5 dt (d - d_obs)^2 and the discrete adjoint;
 - descent direction -dJ/dm, divided by the summed source illumination
-  sum_shots Ws plus 0.005 of its maximum over the window; optionally zero in
+  sum_shots Ws (or sqrt(sum Ws * sum Wr) with the receiver side's energy,
+  precond="source_receiver") plus 0.005 of its maximum over the window; optionally zero in
   the first `mute` rows of the window, where the sources and receivers sit
   and the gradient peaks (a common FWI practice; 0 keeps every row);
+  optionally smoothed by a Gaussian of `smooth` cells (0: not smoothed);
 - the reference's bracketing step search (fwi_step_length_estimation.cu:
   11-224: halve until the misfit drops, then grow by 1.5 until it rises, at
   most 7 tries each) with batched trial forwards, and the vertex of the
@@ -31,6 +33,7 @@ import time
 from dataclasses import dataclass, field
 
 import numpy as np
+from scipy.ndimage import gaussian_filter
 
 from . import psvfwi as P
 
@@ -83,11 +86,17 @@ class Inversion:
     lam, mu, rho on the padded grid; win: (z1, z2, x1, x2), the cells the
     inversion updates; group: shots per batched launch (None: all)."""
 
-    def __init__(self, plan: P.Plan, shots, observed, model, win, segment=100, step=0.01, mute=0, group=None):
+    def __init__(self, plan: P.Plan, shots, observed, model, win, segment=100, step=0.01, mute=0, group=None,
+                 smooth=0.0, precond="source"):
         self.plan = plan
         self.win = win
         self.step = step
         self.mute = int(mute)
+        self.smooth = float(smooth)
+        if precond not in ("source", "source_receiver"):
+            raise ValueError("precond: 'source' or 'source_receiver'")
+        self.precond = precond
+        self.diag = []   # per iteration: where the largest direction component sits, and how peaked it is
         self.m = {"lam": model[0].copy(), "mu": model[1].copy(), "rho": model[2].copy()}
         self.shots = list(shots)
         self.observed = list(observed)
@@ -123,21 +132,30 @@ class Inversion:
         return total
 
     def _gradient(self):
-        """Misfit, summed illumination and dJ/d(lam, mu, rho) of the current model."""
+        """Misfit, illumination and dJ/d(lam, mu, rho) of the current model.  The
+        illumination is the summed source energy sum_shots Ws, or with
+        precond="source_receiver" sqrt(sum Ws * sum Wr), Wr the energy of the
+        adjoint wavefield (the reference's sqrt(Ws Wr) of fd_wave_model.cu:1004-1052
+        on the summed energies; one more adjoint run per group, without the gradient)."""
         self._set(self.m)
-        L2, ill, g = 0.0, 0.0, None
+        L2, ws, wr, g = 0.0, 0.0, 0.0, None
         for gi, ids in enumerate(self.groups):
             self._use(gi)
             self.plan.forward(energy=True, gradient=True)
             self.forwards += 1
             L2 += float(np.sum(self.plan.misfit()))
-            ill = ill + sum(self.plan.field("We", i) for i in range(len(ids)))
+            ws = ws + sum(self.plan.field("We", i) for i in range(len(ids)))
+            if self.precond == "source_receiver":
+                self.plan.adjoint(energy=True, gradient=False)
+                self.adjoints += 1
+                wr = wr + sum(self.plan.field("We", i) for i in range(len(ids)))
             self.plan.adjoint(energy=False, gradient=True)
             self.adjoints += 1
             gg = self.plan.model_gradient(-1)
             g = gg if g is None else [a + b for a, b in zip(g, gg)]
             if self.log and len(self.groups) > 1:
                 self.log(f"  gradient, group {gi + 1}/{len(self.groups)}")
+        ill = np.sqrt(ws * wr) if self.precond == "source_receiver" else ws
         return L2, ill, g
 
     def _trial(self, d, est):
@@ -151,9 +169,17 @@ class Inversion:
         z1, z2, x1, x2 = self.win
         w = ill + 0.005 * ill[z1:z2, x1:x2].max()
         d = {k: -gk / w for k, gk in zip(("lam", "mu", "rho"), g)}
+        if self.smooth > 0:
+            # Gaussian smoothing of the direction over the window (sigma in cells)
+            for k, v in d.items():
+                v[z1:z2, x1:x2] = gaussian_filter(v[z1:z2, x1:x2], self.smooth, mode="nearest")
         if self.mute > 0:
             for v in d.values():
                 v[z1:z1 + self.mute, :] = 0.0
+        a = np.abs(d["lam"][z1:z2, x1:x2])
+        iz, ix = np.unravel_index(int(np.argmax(a)), a.shape)
+        self.diag.append({"lam_max_row": int(iz), "lam_max_col": int(ix),
+                          "lam_max_over_p99": float(a.max() / max(np.percentile(a, 99.0), 1e-300))})
         # the reference's bracketing search
         L2t, sl = [L2, 0.0, 0.0], [0.0, 0.0, 0.0]
         est, step1, step2, itest, count, ntrial = self.step, False, False, 1, 0, 0
@@ -206,8 +232,14 @@ class Inversion:
         m = self.m if m is None else m
         return np.sqrt((m["lam"] + 2.0 * m["mu"]) / m["rho"])
 
-    def model_error(self, truth):
-        """Relative L2 error of vp over the window against truth (lam, mu, rho)."""
+    def model_error(self, truth, sigma=0.0, m=None):
+        """Relative L2 error of vp over the window against truth (lam, mu, rho); with
+        sigma > 0 both vp fields are first low-passed by a Gaussian of sigma cells,
+        which keeps the scales a band-limited inversion can resolve."""
         z1, z2, x1, x2 = self.win
         vt = np.sqrt((truth[0] + 2.0 * truth[1]) / truth[2])[z1:z2, x1:x2]
-        return float(np.linalg.norm(self.vp()[z1:z2, x1:x2] - vt) / np.linalg.norm(vt))
+        v = self.vp(m)[z1:z2, x1:x2]
+        if sigma > 0:
+            vt = gaussian_filter(vt, sigma, mode="nearest")
+            v = gaussian_filter(v, sigma, mode="nearest")
+        return float(np.linalg.norm(v - vt) / np.linalg.norm(vt))

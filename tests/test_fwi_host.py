@@ -14,7 +14,8 @@ from cuda_fwi_amd import psvfwi as P
 class QuadraticPlan:
     """Stands in for psvfwi.Plan: shot s has misfit
     J_s = 0.5 c_s sum_window [((lam - lam*) / 1e9)^2 + ((mu - mu*) / 1e9)^2],
-    its illumination is 1 everywhere, and model_gradient(-1) returns the
+    its source illumination is 1 and its receiver illumination 4 everywhere,
+    and model_gradient(-1) returns the
     exact dJ/d(lam, mu, rho) summed over the current shots (rho: 0).
     `flip` negates the gradient (a search that can only fail)."""
 
@@ -46,6 +47,7 @@ class QuadraticPlan:
 
     def forward(self, energy=True, gradient=False):
         self.calls["forward"] += 1
+        self.energy = 1.0
         dl, dm = self._terms()
         self.J = np.array([0.5 * s["c"] * (np.sum(dl * dl) + np.sum(dm * dm)) for s in self.shots])
 
@@ -54,10 +56,12 @@ class QuadraticPlan:
 
     def field(self, name, shot):
         assert name == "We"
-        return np.ones_like(self.lam)
+        return np.full_like(self.lam, self.energy)
 
     def adjoint(self, energy=False, gradient=True):
         self.calls["adjoint"] += 1
+        if energy:
+            self.energy = 4.0   # the adjoint wavefield's energy: 4 per shot everywhere
 
     def model_gradient(self, shot=-1):
         z1, z2, x1, x2 = self.win
@@ -173,3 +177,55 @@ def test_search_that_cannot_descend_raises():
     inv = _inversion(QuadraticPlan(truth, win, flip=True), start, win, shots, observed)
     with pytest.raises(F.StepSearchFailed):
         inv.iterate()
+
+
+def test_source_receiver_illumination():
+    """precond="source_receiver": one more adjoint run per group for the
+    receiver side's energy; the illumination is sqrt(sum Ws * sum Wr)."""
+    truth, start, win, shots, observed = _case()
+    plan = QuadraticPlan(truth, win)
+    inv = _inversion(plan, start, win, shots, observed, precond="source_receiver", group=2)
+    _, ill, _ = inv._gradient()
+    n = len(shots)
+    assert np.allclose(ill, np.sqrt(n * 1.0 * n * 4.0), rtol=1e-15)
+    assert plan.calls["adjoint"] == 2 * len(inv.groups)
+    plain = QuadraticPlan(truth, win)
+    _, ill1, _ = _inversion(plain, start, win, shots, observed)._gradient()
+    assert np.array_equal(ill1, np.full_like(ill1, n)) and plain.calls["adjoint"] == 1
+    with pytest.raises(ValueError):
+        _inversion(QuadraticPlan(truth, win), start, win, shots, observed, precond="hessian")
+
+
+def test_smoothed_direction_keeps_the_mute_and_converges():
+    """smooth > 0: the direction is low-passed over the window before the muted rows are
+    zeroed; the loop still descends, and the muted rows stay at the start model."""
+    truth, start, win, shots, observed = _case(nz=20, nx=24)
+    z1, z2, x1, x2 = win
+    inv = _inversion(QuadraticPlan(truth, win), start, win, shots, observed, smooth=1.5, mute=3)
+    h = inv.run(4)
+    assert all(b < a for a, b in zip(h.misfit, h.misfit[1:]))
+    for k, i in (("lam", 0), ("mu", 1)):
+        assert np.array_equal(inv.m[k][z1:z1 + 3, x1:x2], start[i][z1:z1 + 3, x1:x2])
+    # the direction's peak is recorded per iteration, inside the window and below the mute
+    assert len(inv.diag) == 4
+    for d in inv.diag:
+        assert 3 <= d["lam_max_row"] < z2 - z1 and 0 <= d["lam_max_col"] < x2 - x1
+        assert d["lam_max_over_p99"] >= 1.0
+
+
+def test_low_pass_model_error():
+    """model_error(truth, sigma): a checkerboard difference (the scale a band-limited
+    inversion cannot resolve) vanishes after the low-pass, a smooth one survives it."""
+    truth, start, win, shots, observed = _case(nz=32, nx=32)
+    z1, z2, x1, x2 = win
+    inv = _inversion(QuadraticPlan(truth, win), start, win, shots, observed)
+    lam, mu, rho = (np.full_like(truth[0], 9.0e9), np.full_like(truth[0], 4.0e9), np.full_like(truth[0], 2200.0))
+    ref = (lam, mu, rho)
+    zz, xx = np.mgrid[0:lam.shape[0], 0:lam.shape[1]]
+    checker = {"lam": lam * (1.0 + 0.02 * (-1.0) ** (zz + xx)), "mu": mu, "rho": rho}
+    smooth = {"lam": lam * (1.0 + 0.02 * np.sin(zz / 40.0)), "mu": mu, "rho": rho}
+    e_c0, e_c4 = inv.model_error(ref, 0.0, checker), inv.model_error(ref, 4.0, checker)
+    e_s0, e_s4 = inv.model_error(ref, 0.0, smooth), inv.model_error(ref, 4.0, smooth)
+    assert e_c0 > 1e-3 and e_c4 < 0.05 * e_c0   # what survives: the sqrt of vp is not linear in lam
+    assert e_s0 > 1e-4 and e_s4 > 0.8 * e_s0
+    assert inv.model_error(ref, 0.0, {"lam": lam, "mu": mu, "rho": rho}) == 0.0
