@@ -207,17 +207,17 @@ def run_c2(args):
     nf = na = 0
     ksum = {"forward_stress": [], "forward_velocity": [], "adjoint_stress": [], "adjoint_velocity": []}
     for i in range(args.steps):
-        (fms, fl, fs_us, fv_us, _), (ams, al, as_us, av_us, _) = one_pass()
+        (fms, fl, fs_us, fv_us, fn), (ams, al, as_us, av_us, an) = one_pass()
         _log(f"timed pass {i}: forward {fms:.1f} ms, adjoint {ams:.1f} ms; stress/velocity kernel "
              f"us: forward {fs_us:.2f}/{fv_us:.2f}, adjoint {as_us:.2f}/{av_us:.2f}")
         tf_ms += fms
         ta_ms += ams
         nf += fl
         na += al
-        ksum["forward_stress"].append(fs_us)
-        ksum["forward_velocity"].append(fv_us)
-        ksum["adjoint_stress"].append(as_us)
-        ksum["adjoint_velocity"].append(av_us)
+        ksum["forward_stress"].append((fs_us, fn))
+        ksum["forward_velocity"].append((fv_us, fn))
+        ksum["adjoint_stress"].append((as_us, an))
+        ksum["adjoint_velocity"].append((av_us, an))
     _sync()
     T = time.perf_counter() - t0
 
@@ -227,7 +227,7 @@ def run_c2(args):
     # the whole time loop, on the plan's stream) split between the two kernels
     # of a step in the ratio of their sampled event-bracketed durations (a
     # single bracketed kernel also carries the event's own dispatch overhead).
-    sampled = {k: float(np.mean(v)) for k, v in ksum.items()}
+    sampled = {k: sum(u * n for u, n in v) / max(sum(n for _, n in v), 1) for k, v in ksum.items()}
     kus = {}
     for kind, ms, n in (("forward", tf_ms, nf), ("adjoint", ta_ms, na)):
         step_us = ms * 1e3 / n
@@ -270,6 +270,11 @@ def run_c2(args):
                      "alg_bytes_per_launch": int(kb), "avg_launch_us": round(kus[kname], 2),
                      "launches": launches[kname],
                      "kernel_us": {k: round(v, 2) for k, v in kus.items()},
+                     "sampled_us_per_launch": {k: round(v, 2) for k, v in sampled.items()},
+                     "duration_note": "avg_launch_us: the loop time per step (HIP events around each time "
+                                      "loop) split between the two kernels in the ratio of their sampled "
+                                      "durations; sampled_us_per_launch: event-bracketed durations of every "
+                                      "64th launch, each with one event's dispatch overhead",
                      "step_alg_GBps": round(60.0 * (esize / 4) * cells * w.nt * 2 * args.steps
                                             / (tf_ms + ta_ms) * 1e3 / 1e9, 1)},
         "forward_ms_per_step": round(tf_ms / args.steps, 2),
@@ -520,6 +525,8 @@ def run_c4(args):
     plan.set_shots([descs[i] for i in groups[0]])
     tile = args.tile if args.tile != 0 else plan.autotune(min(args.tune_steps, w.nt))
     plan.set_tile(tile)
+    if args.shot_order >= 0:
+        plan.set_shot_order(args.shot_order)
     band = plan.shot_order()
     if args.launch_group >= 0:
         plan.set_launch_group(args.launch_group)
@@ -561,7 +568,7 @@ def run_c4(args):
     for i in range(args.steps):
         g = groups[i % len(groups)]
         ts = time.perf_counter()
-        (fms, fl, fs_us, fv_us, _), (ams, al, as_us, av_us, _) = group_step(g)
+        (fms, fl, fs_us, fv_us, fn), (ams, al, as_us, av_us, an) = group_step(g)
         nts = sum(w.shots[j]["nt"] for j in g)
         shots_done += len(g)
         cell_steps += nts * w.interior_cells()
@@ -569,8 +576,8 @@ def run_c4(args):
         loop_ms["adjoint"] += ams
         nlaunch["forward"] += fl
         nlaunch["adjoint"] += al
-        for k, v in zip(kinds, (fs_us, fv_us, as_us, av_us)):
-            samples[k].append(v)
+        for k, v, n in zip(kinds, (fs_us, fv_us, as_us, av_us), (fn, fn, an, an)):
+            samples[k].append((v, n))
             alg_kernel[k] += kb[k] * nts
         _log(f"timed step {i}: group of {len(g)} shots, forward {fms:.0f} ms, adjoint {ams:.0f} ms, "
              f"wall {(time.perf_counter() - ts) * 1e3:.0f} ms")
@@ -580,10 +587,14 @@ def run_c4(args):
     # dominant kernel: each kind's share of its loop time in the ratio of the sampled
     # event-bracketed durations; per launch = that time / launches, algorithmic bytes per
     # launch = the kernel's bytes summed over all shot-steps / launches
+    # the sampled durations themselves: every 64th launch of each timed loop bracketed by
+    # HIP events on the plan's stream, averaged over the samples of all timed steps
+    direct = {k: sum(v * n for v, n in samples[k]) / max(sum(n for _, n in samples[k]), 1) for k in kinds}
+    nsamp = {k: sum(n for _, n in samples[k]) for k in kinds}
     kus_total = {}
     for kind in ("forward", "adjoint"):
-        a = float(np.mean(samples[kind + "_stress"]))
-        b = float(np.mean(samples[kind + "_velocity"]))
+        a = direct[kind + "_stress"]
+        b = direct[kind + "_velocity"]
         kus_total[kind + "_stress"] = loop_ms[kind] * 1e3 * a / (a + b)
         kus_total[kind + "_velocity"] = loop_ms[kind] * 1e3 * b / (a + b)
     launches = {k: nlaunch[k.split("_")[0]] for k in kinds}
@@ -663,7 +674,14 @@ def run_c4(args):
                                        if traffic else None,
                      "alg_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(avg_us, 2),
                      "launches": launches[kname],
-                     "kernel_us_per_launch": {k: round(kus_total[k] / launches[k], 2) for k in kinds}},
+                     "kernel_us_per_launch": {k: round(kus_total[k] / launches[k], 2) for k in kinds},
+                     "sampled_us_per_launch": {k: round(direct[k], 2) for k in kinds},
+                     "samples": nsamp[kname],
+                     "duration_note": "avg_launch_us: the loop time of the timed steps (HIP events around "
+                                      "each time loop) split between the stress and velocity kernels in the "
+                                      "ratio of their sampled durations, so it also carries the launch gaps; "
+                                      "sampled_us_per_launch: the event-bracketed durations of every 64th "
+                                      "launch, each with one event's overhead"},
         "hbm": {"alg_bytes": int(alg_total), "alg_GBps": round(alg_total / T / 1e9, 1),
                 "frac_of_peak": round(alg_total / T / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "algorithmic bytes of forward + adjoint + gradient of every timed step over the "
@@ -817,6 +835,8 @@ def main():
                     help="replay the time loop as captured HIP graphs of this many steps (0: direct launches)")
     ap.add_argument("--gradient", default="reference", choices=["reference", "discrete"],
                     help="default workload: the reference's correlation gradient or the discrete (exact) one")
+    ap.add_argument("--shot-order", type=int, default=-1,
+                    help="c4: block order, 0 shot-major or bands of this many tile rows (-1: autotuned)")
     ap.add_argument("--launch-group", type=int, default=6,
                     help="c4: shots per launch group (-1: the autotuner's choice, 0: whole group); 6 was the "
                          "fastest over whole runs (the autotuner times short runs)")
