@@ -382,7 +382,8 @@ def run_c5(args):
     start = w.padded(*w.background)
     win = (w.zoff, w.zoff + nz, w.ppad, w.ppad + nx)
     inv = Inversion(plan, shots, observed, start, win, segment=args.interval, mute=args.c5_mute,
-                    group=args.c5_group or None, smooth=args.c5_smooth * sc, precond=args.c5_precond)
+                    group=args.c5_group or None, smooth=args.c5_smooth * sc, precond=args.c5_precond,
+                    gradient=args.c5_gradient, fwi_dt=args.fwi_dt)
     err0 = inv.model_error(truth)
     # vp errors after low-passing both models: 50 m and 150 m (the start model's smoothing) at full scale
     sigmas = (10.0 * sc, 30.0 * sc)
@@ -424,7 +425,10 @@ def run_c5(args):
         "data": "synthetic: 10 layers, two faults and a vp 4500 irregular body (seed 7); start model = "
                 "truth smoothed over 150 m (scaled)",
         "config": {"workload": f"FWI {nx}x{nz} dx5 order-4 npml40, {nshots} shots x {nrec} receivers, nt {nt}, "
-                               f"8 Hz, {args.iters} iterations, discrete gradient, checkpoints every "
+                               f"8 Hz, {args.iters} iterations, "
+                               + ("discrete gradient" if args.c5_gradient == "discrete" else
+                                  f"reference correlation gradient (fwi_dt {args.fwi_dt})")
+                               + ", checkpoints every "
                                f"{args.interval} steps, top {args.c5_mute} model rows not updated"
                                + (f", direction smoothed over {args.c5_smooth:g} cells (full scale)"
                                   if args.c5_smooth > 0 else "")
@@ -810,6 +814,8 @@ def main():
                     help="c5: Gaussian smoothing of the descent direction, sigma in cells at full scale (0: none)")
     ap.add_argument("--c5-precond", default="source", choices=["source", "source_receiver"],
                     help="c5: illumination that divides the gradient: source energy, or sqrt(source x receiver)")
+    ap.add_argument("--c5-gradient", default="discrete", choices=["discrete", "reference"],
+                    help="c5: the exact gradient of the scheme, or the reference's correlation gradient")
     ap.add_argument("--c5-resume", default="",
                     help="c5: continue from a state file written by --c5-save")
     ap.add_argument("--c5-save", default="", help="c5: write the model, step and history here after the run")

@@ -6,7 +6,9 @@ with the batched, window-correct choices of the C++ driver
 (cuda_fwi_amd/csrc/host/fd_wave_model.cpp, ``[mi355x] gradient = discrete``):
 
 - per iteration one batched forward with the energy weights, the FP64 misfit
-  sum_shots sum_rec sum_t 0.5 dt (d - d_obs)^2 and the discrete adjoint;
+  sum_shots sum_rec sum_t 0.5 dt (d - d_obs)^2 and the discrete adjoint (or,
+  with gradient="reference", the reference's adjoint with the correlation
+  gradient over the model window, forward states from checkpoints);
 - descent direction -dJ/dm, divided by the summed source illumination
   sum_shots Ws (or sqrt(sum Ws * sum Wr) with the receiver side's energy,
   precond="source_receiver") plus 0.005 of its maximum over the window; optionally zero in
@@ -87,7 +89,7 @@ class Inversion:
     inversion updates; group: shots per batched launch (None: all)."""
 
     def __init__(self, plan: P.Plan, shots, observed, model, win, segment=100, step=0.01, mute=0, group=None,
-                 smooth=0.0, precond="source"):
+                 smooth=0.0, precond="source", gradient="discrete", fwi_dt=5):
         self.plan = plan
         self.win = win
         self.step = step
@@ -105,7 +107,18 @@ class Inversion:
         self.groups = [list(range(i, min(i + g, n))) for i in range(0, n, g)]
         self.current = None
         self.log = None   # progress callback: one line per group of a gradient or trial
-        plan.set_gradient_mode("discrete", segment)
+        if gradient not in ("discrete", "reference"):
+            raise ValueError("gradient: 'discrete' or 'reference'")
+        self.gradient = gradient
+        if gradient == "discrete":
+            plan.set_gradient_mode("discrete", segment)
+        else:
+            # the reference's correlation gradient over the model window, every fwi_dt-th
+            # step, the forward state from checkpoints every `segment` steps
+            z1, z2, x1, x2 = win
+            plan.set_gradient_mode("reference")
+            plan.set_store_mode("checkpoint", segment)
+            plan.set_window(dict(dt=int(fwi_dt), dx=1, dz=1, x1=x1, x2=x2, z1=z1, z2=z2))
         self.forwards = 0
         self.adjoints = 0
 
@@ -136,7 +149,8 @@ class Inversion:
         illumination is the summed source energy sum_shots Ws, or with
         precond="source_receiver" sqrt(sum Ws * sum Wr), Wr the energy of the
         adjoint wavefield (the reference's sqrt(Ws Wr) of fd_wave_model.cu:1004-1052
-        on the summed energies; one more adjoint run per group, without the gradient)."""
+        on the summed energies; with the discrete gradient one more adjoint run per
+        group, without the gradient, since the discrete adjoint keeps no energy)."""
         self._set(self.m)
         L2, ws, wr, g = 0.0, 0.0, 0.0, None
         for gi, ids in enumerate(self.groups):
@@ -145,18 +159,37 @@ class Inversion:
             self.forwards += 1
             L2 += float(np.sum(self.plan.misfit()))
             ws = ws + sum(self.plan.field("We", i) for i in range(len(ids)))
-            if self.precond == "source_receiver":
+            sr = self.precond == "source_receiver"
+            if sr and self.gradient == "discrete":
+                # the discrete adjoint keeps no energy: an adjoint run without the gradient
                 self.plan.adjoint(energy=True, gradient=False)
                 self.adjoints += 1
                 wr = wr + sum(self.plan.field("We", i) for i in range(len(ids)))
-            self.plan.adjoint(energy=False, gradient=True)
+            self.plan.adjoint(energy=sr and self.gradient == "reference", gradient=True)
             self.adjoints += 1
-            gg = self.plan.model_gradient(-1)
+            if sr and self.gradient == "reference":
+                wr = wr + sum(self.plan.field("We", i) for i in range(len(ids)))
+            gg = self._model_gradient()
             g = gg if g is None else [a + b for a, b in zip(g, gg)]
             if self.log and len(self.groups) > 1:
                 self.log(f"  gradient, group {gi + 1}/{len(self.groups)}")
         ill = np.sqrt(ws * wr) if self.precond == "source_receiver" else ws
         return L2, ill, g
+
+    def _model_gradient(self):
+        """dJ/d(lam, mu, rho) of the current group on the padded grid.  The reference's
+        correlation gradient approximates -dJ/dm on the window (its [nfz][nfx] arrays
+        carry one more row and column, never written): negated and placed on the grid,
+        zero outside the window."""
+        if self.gradient == "discrete":
+            return self.plan.model_gradient(-1)
+        z1, z2, x1, x2 = self.win
+        out = []
+        for gw in self.plan.gradient(-1):
+            a = np.zeros_like(self.m["lam"])
+            a[z1:z2, x1:x2] = -gw[:z2 - z1, :x2 - x1]
+            out.append(a)
+        return out
 
     def _trial(self, d, est):
         m = self.m

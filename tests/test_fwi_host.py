@@ -26,8 +26,26 @@ class QuadraticPlan:
         self.mode = None
         self.calls = {"forward": 0, "adjoint": 0}
 
-    def set_gradient_mode(self, mode, segment):
+    def set_gradient_mode(self, mode, segment=64):
         self.mode = (mode, segment)
+
+    def set_store_mode(self, mode, interval=0):
+        self.store = (mode, interval)
+
+    def set_window(self, win):
+        self.window = win
+
+    def gradient(self, shot=-1):
+        """The reference's window gradient: -dJ/dm on [nfz][nfx] = window + 1 row and column
+        (that row and column never written)."""
+        z1, z2, x1, x2 = self.win
+        g = self.model_gradient(shot)
+        out = []
+        for a in g:
+            w = np.zeros((z2 - z1 + 1, x2 - x1 + 1))
+            w[:-1, :-1] = -a[z1:z2, x1:x2]
+            out.append(w)
+        return out
 
     def set_shots(self, shots):
         self.shots = list(shots)
@@ -229,3 +247,22 @@ def test_low_pass_model_error():
     assert e_c0 > 1e-3 and e_c4 < 0.05 * e_c0   # what survives: the sqrt of vp is not linear in lam
     assert e_s0 > 1e-4 and e_s4 > 0.8 * e_s0
     assert inv.model_error(ref, 0.0, {"lam": lam, "mu": mu, "rho": rho}) == 0.0
+
+
+def test_reference_gradient_mode_equals_the_discrete_loop_on_the_same_gradient():
+    """gradient="reference": the window gradient is negated and placed on the grid; with a
+    plan whose reference gradient is exactly -dJ/dm the loop takes the discrete loop's
+    iterations, and the plan is put into the checkpoint store mode over the model window."""
+    truth, start, win, shots, observed = _case()
+    a = _inversion(QuadraticPlan(truth, win), start, win, shots, observed)
+    pb = QuadraticPlan(truth, win)
+    b = _inversion(pb, start, win, shots, observed, gradient="reference", fwi_dt=3)
+    z1, z2, x1, x2 = win
+    assert pb.mode[0] == "reference" and pb.store == ("checkpoint", 40)
+    assert pb.window == dict(dt=3, dx=1, dz=1, x1=x1, x2=x2, z1=z1, z2=z2)
+    ha, hb = a.run(3), b.run(3)
+    assert ha.misfit == hb.misfit and ha.step == hb.step
+    for k in a.m:
+        assert np.array_equal(a.m[k], b.m[k])
+    with pytest.raises(ValueError):
+        _inversion(QuadraticPlan(truth, win), start, win, shots, observed, gradient="adjoint")
