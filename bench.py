@@ -688,8 +688,13 @@ def run_c4(args):
                                       "launch, each with one event's overhead"},
         "hbm": {"alg_bytes": int(alg_total), "alg_GBps": round(alg_total / T / 1e9, 1),
                 "frac_of_peak": round(alg_total / T / 1e9 / HBM_PEAK_GBS, 4),
+                "in_time_loops": round((loop_ms["forward"] + loop_ms["adjoint"]) / 1e3 / T, 4),
+                "frac_of_peak_in_time_loops": round(alg_total / ((loop_ms["forward"] + loop_ms["adjoint"]) / 1e3)
+                                                    / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "algorithmic bytes of forward + adjoint + gradient of every timed step over the "
-                        "wall time of the timed steps"},
+                        "wall time of the timed steps; in_time_loops: the share of that wall time inside "
+                        "the forward and adjoint time loops (HIP events), the rest being shot tables, "
+                        "observed-trace uploads and the misfit"},
         "batched_shots_per_s": round(batched_rate, 3),
         "per_shot_shots_per_s": round(per_rate, 3),
         "per_shot_gcell_updates_per_s": round(per_gcell, 3),

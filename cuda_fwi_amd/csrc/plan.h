@@ -410,6 +410,10 @@ private:
     DBuf<int> it_dev_;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // the z-marching step's edge segments run on a second stream (PSVFWI_MARCH_STREAMS=1: on
+    // the plan's stream after the interior ones)
+    psv::MarchSide side_{};
+    bool use_side_ = true;
 
     // model and CPML profiles
     DBuf<T> lam_, mu_, mu_zx_, rho_zp_, rho_xp_, pml_;

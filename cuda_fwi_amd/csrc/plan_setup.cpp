@@ -1,5 +1,7 @@
 // plan_setup.cpp -- construction of a plan, its configuration (model, CPML, window, store and
 // gradient modes, tiles, launch groups) and its state and store buffers.
+#include <cstdlib>
+
 #include "plan.h"
 
 namespace psvplan {
@@ -34,6 +36,9 @@ Plan<T>::~Plan()
             if (sample_ev_[i][j]) (void)hipEventDestroy(sample_ev_[i][j]);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (side_.fork) (void)hipEventDestroy(side_.fork);
+    if (side_.join) (void)hipEventDestroy(side_.join);
+    if (side_.stream) (void)hipStreamDestroy(side_.stream);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -41,6 +46,13 @@ template <typename T>
 psvfwi_status Plan<T>::init()
 {
     PSV_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    PSV_HIP(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking));
+    PSV_HIP(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming));
+    PSV_HIP(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming));
+    {
+        const char* e = std::getenv("PSVFWI_MARCH_STREAMS");
+        use_side_ = !(e && std::string(e) == "1");
+    }
     PSV_HIP(hipEventCreate(&ev0_));
     PSV_HIP(hipEventCreate(&ev1_));
     PSV_HIP(pml_.alloc((size_t)6 * tp_));

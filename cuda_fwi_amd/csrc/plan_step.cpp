@@ -180,7 +180,8 @@ psvfwi_status Plan<T>::launch_plain(psv::StepParams<T> p, int mode, State<T>& st
     p.mvzx_w = alt.ms[1].p;
     p.mvxz_w = alt.ms[2].p;
     p.mvzz_w = alt.ms[3].p;
-    if (tz_abs() >= 32) PSV_HIP(psv::launch_march<T>(p, g_.fdorder, mode, tz_abs(), stream_));
+    if (tz_abs() >= 32)
+        PSV_HIP(psv::launch_march<T>(p, g_.fdorder, mode, tz_abs(), stream_, use_side_ ? &side_ : nullptr));
     else PSV_HIP(psv::launch_fused<T>(p, g_.fdorder, mode, tz_abs(), stream_));
     if (e_mid) PSV_HIP(hipEventRecord(e_mid, stream_));
     for (int f = 0; f < 5; f++) swap_buf(st.f[f], alt.f[f]);

@@ -259,25 +259,29 @@ __device__ __forceinline__ void march_segment(const StepParams<T>& p, MarchLds<T
         T rxp, rzp, we;                        // velocity row
     };
     const int hc = tx < 2 ? x0 - 2 + tx : x0 + 62 + tx;  // halo column loaded by lanes 0..3
-    // rows past the last one a segment needs (velocity rows ze+3, stress rows ze+1) are not
-    // read: the bottom segment's would lie beyond the guard rows
+    // Every load of an iteration is issued on every path, so that the number of memory
+    // operations between a register set's loads and its use is the same on all paths and
+    // the wait before the use leaves the next set's loads in flight; nothing is computed
+    // from a loaded value here (a select would wait for it at once).  A row outside the rows
+    // an iteration uses is read at the nearest row it may read (velocity ring rows up to
+    // ze + 3, stress rows up to ze + 1: the bottom segment's later rows would lie beyond the
+    // guard rows; the segment's own rows for the rest), and its value is never used: stress
+    // rows below ze + 1 and their velocity ring rows feed no velocity row, and the other
+    // operands are used for the segment's own rows only.  Lanes 4..63 read their own column
+    // in place of a halo column.
     auto load_ops = [&](int i, Ops& q) {
         const int zr = zs - 2 + R * i;
-        q.vx = q.vz = q.vxh = q.vzh = 0;
-        q.sxx = q.szx = q.szz = q.lam = q.mu = q.mzx = 0;
-        if (zr + 2 + wv <= ze + 3) {
-            const int gz = zr + 2 + wv;
+        {
+            const int gz = min(zr + 2 + wv, ze + 3);
             const unsigned o = boff(gz, x0 + tx);
+            const unsigned oh = tx < 4 ? boff(gz, hc) : o;
             q.vx = ldm(vxA, o);
             q.vz = ldm(vzA, o);
-            if (tx < 4) {
-                const unsigned oh = boff(gz, hc);
-                q.vxh = ldm(vxA, oh);
-                q.vzh = ldm(vzA, oh);
-            }
+            q.vxh = ldm(vxA, oh);
+            q.vzh = ldm(vzA, oh);
         }
-        if (zr + wv <= ze + 1) {
-            const int gz = zr + wv;
+        {
+            const int gz = min(zr + wv, ze + 1);
             const unsigned o = boff(gz, x0 + tx);
             q.sxx = ldm(sxxA, o);
             q.szx = ldm(szxA, o);
@@ -285,21 +289,17 @@ __device__ __forceinline__ void march_segment(const StepParams<T>& p, MarchLds<T
             q.lam = ldm(lamM, o);
             q.mu = ldm(muM, o);
             q.mzx = ldm(mzxM, o);
-            q.ux = q.uz = 0;
-            if (MODE == kAdjoint && gz >= zs && gz < ze) {
-                q.ux = ldm((const T*)uxS, o);
-                q.uz = ldm((const T*)uzS, o);
+            if (MODE == kAdjoint) {
+                const unsigned oo = boff(max(zs, min(zr + wv, ze - 1)), x0 + tx);
+                q.ux = ldm((const T*)uxS, oo);
+                q.uz = ldm((const T*)uzS, oo);
             }
         }
         {
-            const int gz = zr - 2 + wv;
-            const unsigned o = boff(gz, x0 + tx);
-            q.rxp = q.rzp = q.we = 0;
-            if (gz >= zs && gz < ze) {
-                q.rxp = ldm(rxpM, o);
-                q.rzp = ldm(rzpM, o);
-                if (energy) q.we = ldm((const T*)WeS, o);
-            }
+            const unsigned o = boff(max(zs, min(zr - 2 + wv, ze - 1)), x0 + tx);
+            q.rxp = ldm(rxpM, o);
+            q.rzp = ldm(rzpM, o);
+            if (energy) q.we = ldm((const T*)WeS, o);
         }
     };
 
@@ -319,9 +319,9 @@ __device__ __forceinline__ void march_segment(const StepParams<T>& p, MarchLds<T
     }
     // operands are loaded two iterations ahead (two register sets in flight), so a
     // workgroup waits on memory once per two iterations at most
-    Ops qa, qb;
+    Ops qa{}, qb{};
     load_ops(0, qa);
-    if (niter > 1) load_ops(1, qb);
+    load_ops(1, qb);  // unconditional (its rows are clamped): see the loop below
     auto iteration = [&](int i, const Ops& cu) {
         const int zr = zs - 2 + R * i;
         {
@@ -532,18 +532,26 @@ __device__ __forceinline__ void march_segment(const StepParams<T>& p, MarchLds<T
             }
         }
     };
-    for (int i = 0; i < niter; i += 2) {
-        {
-            const Ops cu = qa;
-            if (i + 2 < niter) load_ops(i + 2, qa);
-            iteration(i, cu);
-        }
-        if (i + 1 < niter) {
-            const Ops cu = qb;
-            if (i + 3 < niter) load_ops(i + 3, qb);
-            iteration(i + 1, cu);
-        }
+    // Each register set is consumed, then refilled for the iteration two ahead: no set is
+    // copied while its loads are in flight (a copy of registers still being loaded makes the
+    // compiler wait for every outstanding load at the copy), so the loads for i + 2 stay in
+    // flight through iteration i + 1.  The loop runs while both refills happen, so every
+    // path through it issues the same loads (a path that skips one would make the compiler
+    // wait for all loads in flight at the loop head); the last one to three iterations
+    // follow it.
+    int i = 0;
+    for (; i + 3 < niter; i += 2) {
+        iteration(i, qa);
+        load_ops(i + 2, qa);
+        iteration(i + 1, qb);
+        load_ops(i + 3, qb);
     }
+    if (i < niter) {
+        iteration(i, qa);
+        if (i + 2 < niter) load_ops(i + 2, qa);
+    }
+    if (i + 1 < niter) iteration(i + 1, qb);
+    if (i + 2 < niter) iteration(i + 2, qa);
 }
 
 // PATH 0: every segment of the launch, interior ones on the lean path; 1: interior segments
@@ -604,13 +612,28 @@ __global__ __launch_bounds__(256) void psv_point_kernel(const StepParams<T> p)
 }
 
 template <typename T, int ORDER, int MODE, int S>
-static hipError_t launch_march_one(const StepParams<T>& p, hipStream_t s)
+static hipError_t launch_march_one(const StepParams<T>& p, hipStream_t s, const MarchSide* side)
 {
     const int blocks = p.tiles_x * p.tiles_z * p.nslots;
     // interior segments on the lean instance (no boundary code: a third of the registers,
-    // twice the waves per SIMD), then the segments along the grid's edges
-    hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 1>), dim3(blocks), dim3(kThreads), 0, s, p);
-    hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 2>), dim3(blocks), dim3(kThreads), 0, s, p);
+    // twice the waves per SIMD) and the segments along the grid's edges on the full one;
+    // both read only the old buffer set and write disjoint cells of the new one, so with a
+    // second stream they run side by side, and the edge segments' long serial walk overlaps
+    // the interior launch
+    if (side) {
+        hipError_t e = hipEventRecord(side->fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side->stream, side->fork, 0);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 2>), dim3(blocks), dim3(kThreads), 0,
+                           side->stream, p);
+        hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 1>), dim3(blocks), dim3(kThreads), 0, s, p);
+        e = hipEventRecord(side->join, side->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
+        if (e != hipSuccess) return e;
+    } else {
+        hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 1>), dim3(blocks), dim3(kThreads), 0, s, p);
+        hipLaunchKernelGGL((psv_march_kernel<T, ORDER, MODE, S, 2>), dim3(blocks), dim3(kThreads), 0, s, p);
+    }
     hipLaunchKernelGGL((psv_point_kernel<T>), dim3(p.nslots), dim3(256), 0, s, p);
     if (p.nexpl > 0) {
         StepParams<T> q = p;
@@ -623,10 +646,10 @@ static hipError_t launch_march_one(const StepParams<T>& p, hipStream_t s)
 }
 
 #define PSV_MCASE(O, M, Z) \
-    if (order == O && mode == M && rows == Z) return launch_march_one<T, O, M, Z>(p, s);
+    if (order == O && mode == M && rows == Z) return launch_march_one<T, O, M, Z>(p, s, side);
 
 template <typename T>
-hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, hipStream_t s)
+hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, hipStream_t s, const MarchSide* side)
 {
     PSV_MCASE(2, kForward, 32) PSV_MCASE(2, kAdjoint, 32)
     PSV_MCASE(4, kForward, 32) PSV_MCASE(4, kAdjoint, 32)
@@ -636,9 +659,9 @@ hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, h
 }
 #undef PSV_MCASE
 
-template hipError_t launch_march<float>(const StepParams<float>&, int, int, int, hipStream_t);
-template hipError_t launch_march<double>(const StepParams<double>&, int, int, int, hipStream_t);
-template hipError_t launch_march<dfloat>(const StepParams<dfloat>&, int, int, int, hipStream_t);
-template hipError_t launch_march<ffloat>(const StepParams<ffloat>&, int, int, int, hipStream_t);
+template hipError_t launch_march<float>(const StepParams<float>&, int, int, int, hipStream_t, const MarchSide*);
+template hipError_t launch_march<double>(const StepParams<double>&, int, int, int, hipStream_t, const MarchSide*);
+template hipError_t launch_march<dfloat>(const StepParams<dfloat>&, int, int, int, hipStream_t, const MarchSide*);
+template hipError_t launch_march<ffloat>(const StepParams<ffloat>&, int, int, int, hipStream_t, const MarchSide*);
 
 }  // namespace psv

@@ -126,7 +126,16 @@ hipError_t launch_fused(const StepParams<T>& p, int order, int mode, int tz, hip
 // rings, followed by the point-source / receiver launch (and the explosive
 // one when needed); reads vx.. and writes vx_w.. like launch_fused.  Returns
 // hipErrorInvalidValue for other segment heights.
+// The edge segments of a z-marching step run on a second stream, concurrently with
+// the interior ones: `fork` is recorded on the step's stream and waited for on
+// `stream`, `join` the other way round, before the point-source launch.
+struct MarchSide {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+};
+
 template <typename T>
-hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, hipStream_t s);
+hipError_t launch_march(const StepParams<T>& p, int order, int mode, int rows, hipStream_t s,
+                        const MarchSide* side = nullptr);
 
 }  // namespace psv
