@@ -383,7 +383,8 @@ def run_c5(args):
     win = (w.zoff, w.zoff + nz, w.ppad, w.ppad + nx)
     inv = Inversion(plan, shots, observed, start, win, segment=args.interval, mute=args.c5_mute,
                     group=args.c5_group or None, smooth=args.c5_smooth * sc, precond=args.c5_precond,
-                    gradient=args.c5_gradient, fwi_dt=args.fwi_dt)
+                    gradient=args.c5_gradient, fwi_dt=args.fwi_dt, taper=args.c5_taper,
+                    depth_power=args.c5_depth_power)
     err0 = inv.model_error(truth)
     # vp errors after low-passing both models: 50 m and 150 m (the start model's smoothing) at full scale
     sigmas = (10.0 * sc, 30.0 * sc)
@@ -432,7 +433,9 @@ def run_c5(args):
                                f"{args.interval} steps, top {args.c5_mute} model rows not updated"
                                + (f", direction smoothed over {args.c5_smooth:g} cells (full scale)"
                                   if args.c5_smooth > 0 else "")
-                               + (", preconditioner sqrt(Ws Wr)" if args.c5_precond == "source_receiver" else ""),
+                               + (", preconditioner sqrt(Ws Wr)" if args.c5_precond == "source_receiver" else "")
+                               + (f", cosine taper over {args.c5_taper} rows" if args.c5_taper > 0 else "")
+                               + (f", depth gain power {args.c5_depth_power:g}" if args.c5_depth_power > 0 else ""),
                    "scale_of_2400x800_128_shots": sc, "tile": plan.tile,
                    "launch_group": plan.launch_group() or "whole batch"},
         "fwi": {"misfit": prev["misfit"], "vp_rel_error_start": err0, "vp_rel_error": prev["model_error"],
@@ -821,6 +824,10 @@ def main():
                     help="c5: illumination that divides the gradient: source energy, or sqrt(source x receiver)")
     ap.add_argument("--c5-gradient", default="discrete", choices=["discrete", "reference"],
                     help="c5: the exact gradient of the scheme, or the reference's correlation gradient")
+    ap.add_argument("--c5-taper", type=int, default=0,
+                    help="c5: rows below the muted ones over which the direction ramps up (cosine)")
+    ap.add_argument("--c5-depth-power", type=float, default=0.0,
+                    help="c5: depth gain of the direction, ((row + 1) / rows)^p (0: none)")
     ap.add_argument("--c5-resume", default="",
                     help="c5: continue from a state file written by --c5-save")
     ap.add_argument("--c5-save", default="", help="c5: write the model, step and history here after the run")

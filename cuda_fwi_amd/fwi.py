@@ -14,7 +14,9 @@ with the batched, window-correct choices of the C++ driver
   precond="source_receiver") plus 0.005 of its maximum over the window; optionally zero in
   the first `mute` rows of the window, where the sources and receivers sit
   and the gradient peaks (a common FWI practice; 0 keeps every row);
-  optionally smoothed by a Gaussian of `smooth` cells (0: not smoothed);
+  optionally smoothed by a Gaussian of `smooth` cells (0: not smoothed),
+  scaled by a depth gain ((row + 1) / rows)^depth_power, and ramped up by a
+  cosine over the `taper` rows below the muted ones;
 - the reference's bracketing step search (fwi_step_length_estimation.cu:
   11-224: halve until the misfit drops, then grow by 1.5 until it rises, at
   most 7 tries each) with batched trial forwards, and the vertex of the
@@ -89,12 +91,14 @@ class Inversion:
     inversion updates; group: shots per batched launch (None: all)."""
 
     def __init__(self, plan: P.Plan, shots, observed, model, win, segment=100, step=0.01, mute=0, group=None,
-                 smooth=0.0, precond="source", gradient="discrete", fwi_dt=5):
+                 smooth=0.0, precond="source", gradient="discrete", fwi_dt=5, taper=0, depth_power=0.0):
         self.plan = plan
         self.win = win
         self.step = step
         self.mute = int(mute)
         self.smooth = float(smooth)
+        self.taper = int(taper)
+        self.depth_power = float(depth_power)
         if precond not in ("source", "source_receiver"):
             raise ValueError("precond: 'source' or 'source_receiver'")
         self.precond = precond
@@ -206,9 +210,22 @@ class Inversion:
             # Gaussian smoothing of the direction over the window (sigma in cells)
             for k, v in d.items():
                 v[z1:z2, x1:x2] = gaussian_filter(v[z1:z2, x1:x2], self.smooth, mode="nearest")
+        if self.depth_power > 0:
+            # depth gain: row r of the window (r = 0 at its top) times ((r + 1) / rows)^p
+            g = ((np.arange(z2 - z1) + 1.0) / (z2 - z1)) ** self.depth_power
+            for v in d.values():
+                v[z1:z2, :] *= g[:, None]
         if self.mute > 0:
             for v in d.values():
                 v[z1:z1 + self.mute, :] = 0.0
+        if self.taper > 0:
+            # a cosine ramp from 0 to 1 over the `taper` rows below the muted ones
+            r = np.arange(self.taper)
+            g = 0.5 * (1.0 - np.cos(np.pi * (r + 0.5) / self.taper))
+            a0 = z1 + self.mute
+            n = min(self.taper, z2 - a0)
+            for v in d.values():
+                v[a0:a0 + n, :] *= g[:n, None]
         a = np.abs(d["lam"][z1:z2, x1:x2])
         iz, ix = np.unravel_index(int(np.argmax(a)), a.shape)
         self.diag.append({"lam_max_row": int(iz), "lam_max_col": int(ix),

@@ -266,3 +266,34 @@ def test_reference_gradient_mode_equals_the_discrete_loop_on_the_same_gradient()
         assert np.array_equal(a.m[k], b.m[k])
     with pytest.raises(ValueError):
         _inversion(QuadraticPlan(truth, win), start, win, shots, observed, gradient="adjoint")
+
+
+def _first_update(**kw):
+    truth, start, win, shots, observed = _case(nz=24, nx=20)
+    inv = _inversion(QuadraticPlan(truth, win), start, win, shots, observed, **kw)
+    inv.iterate()
+    return inv.m["lam"] - start[0], win
+
+
+def test_depth_gain_and_taper_scale_the_rows():
+    """depth_power p multiplies row r of the window (r = 0 at its top) by ((r + 1) / rows)^p,
+    and taper t ramps the t rows below the muted ones by 0.5 (1 - cos(pi (k + 0.5) / t)):
+    between two rows of one update the ratio of the model changes is the unscaled ratio
+    times the ratio of those weights, whatever step the search takes."""
+    d0, win = _first_update()
+    z1, z2, x1, x2 = win
+    rows = z2 - z1
+    c = x1 + 7
+    for kw, weight in (
+        (dict(depth_power=2.0), lambda r: ((r + 1.0) / rows) ** 2),
+        (dict(mute=3, taper=6), lambda r: 0.0 if r < 3 else (0.5 * (1 - np.cos(np.pi * (r - 3 + 0.5) / 6))
+                                                              if r < 9 else 1.0)),
+    ):
+        d1, _ = _first_update(**kw)
+        ra, rb = 12, 5
+        assert abs(d0[z1 + ra, c]) > 0 and abs(d0[z1 + rb, c]) > 0
+        got = (d1[z1 + rb, c] / d1[z1 + ra, c]) / (d0[z1 + rb, c] / d0[z1 + ra, c])
+        assert got == pytest.approx(weight(rb) / weight(ra), rel=1e-9), kw
+    # muted rows stay, and a taper row moves less than the rows below the taper
+    d2, _ = _first_update(mute=3, taper=6)
+    assert np.all(d2[z1:z1 + 3, x1:x2] == 0.0)
