@@ -515,6 +515,11 @@ void MI355XOptions::parse(const char* path)
     gradient_scaling = r.Get(s, "gradient_scaling", gradient_scaling);
     gradient = r.Get(s, "gradient", gradient);
     gradient_segment = (int)r.GetInteger(s, "gradient_segment", gradient_segment);
+    illumination = r.Get(s, "illumination", illumination);
+    direction_smooth = r.GetReal(s, "direction_smooth", direction_smooth);
+    direction_depth_power = r.GetReal(s, "direction_depth_power", direction_depth_power);
+    direction_mute = (int)r.GetInteger(s, "direction_mute", direction_mute);
+    direction_taper = (int)r.GetInteger(s, "direction_taper", direction_taper);
     output_dir = r.Get(s, "output_dir", output_dir);
     write_csv = r.GetBoolean(s, "write_csv", write_csv);
     if (precision != "fp32" && precision != "fp64" && precision != "fp32x2" && precision != "fp32fma")
@@ -531,4 +536,9 @@ void MI355XOptions::parse(const char* path)
     if (gradient != "reference" && gradient != "discrete")
         throw std::runtime_error("[mi355x] gradient: reference or discrete");
     if (gradient_segment < 1) throw std::runtime_error("[mi355x] gradient_segment must be >= 1");
+    if (illumination != "source" && illumination != "source_receiver")
+        throw std::runtime_error("[mi355x] illumination: source or source_receiver");
+    if (!(direction_smooth >= 0.0) || !(direction_depth_power >= 0.0) || direction_mute < 0 || direction_taper < 0)
+        throw std::runtime_error("[mi355x] direction_smooth, direction_depth_power, direction_mute and "
+                                 "direction_taper must be >= 0");
 }

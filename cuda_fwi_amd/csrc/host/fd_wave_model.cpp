@@ -497,12 +497,57 @@ real_sim FDWaveModel::fwi_step_length_estimation(real_sim est, real_sim* L2_test
     return calc_opt_step(L2_test, step_length);
 }
 
+// Gaussian smoothing of one [nz][nx] array in place: a separable correlation with the
+// normalised weights exp(-k^2 / (2 sigma^2)), |k| <= r = int(4 sigma + 0.5), rows first, then
+// columns, indices beyond an edge clamped to it (the definition of scipy.ndimage's
+// gaussian_filter with mode "nearest" and truncate 4).
+static void gaussian_smooth(double* a, int nz, int nx, double sigma)
+{
+    const int r = (int)(4.0 * sigma + 0.5);
+    std::vector<double> w((size_t)(2 * r + 1));
+    double sum = 0.0;
+    for (int k = -r; k <= r; k++) sum += (w[(size_t)(k + r)] = std::exp(-0.5 * k * k / (sigma * sigma)));
+    for (double& v : w) v /= sum;
+    std::vector<double> t((size_t)nz * nx);
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+            double acc = 0.0;
+            for (int k = -r; k <= r; k++) acc += w[(size_t)(k + r)] * a[(size_t)std::min(std::max(z + k, 0), nz - 1) * nx + x];
+            t[(size_t)z * nx + x] = acc;
+        }
+    for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+            double acc = 0.0;
+            for (int k = -r; k <= r; k++) acc += w[(size_t)(k + r)] * t[(size_t)z * nx + std::min(std::max(x + k, 0), nx - 1)];
+            a[(size_t)z * nx + x] = acc;
+        }
+}
+
+void shape_direction(double* d[3], int nfz, int nfx, const MI355XOptions& o)
+{
+    for (int k = 0; k < 3; k++) {
+        double* a = d[k];
+        if (o.direction_smooth > 0.0) gaussian_smooth(a, nfz, nfx, o.direction_smooth);
+        for (int z = 0; z < nfz; z++) {
+            double g = 1.0;
+            if (o.direction_depth_power > 0.0) g = std::pow((z + 1.0) / nfz, o.direction_depth_power);
+            if (z < o.direction_mute) g = 0.0;
+            else if (z < o.direction_mute + o.direction_taper)
+                g *= 0.5 * (1.0 - std::cos(M_PI * (z - o.direction_mute + 0.5) / o.direction_taper));
+            if (g != 1.0)
+                for (int x = 0; x < nfx; x++) a[(size_t)z * nfx + x] *= g;
+        }
+    }
+}
+
 // Discrete gradient (an extension): the descent direction on the FWI window
-// is -dJ/dm of the batch, divided by the source illumination sum_shots Ws
-// plus 0.005 of its maximum over the window (a positive diagonal
-// preconditioner, in place of the reference's energy scaling).
-static void discrete_direction(const FDWaveModel& m, psvfwi_plan* p, const std::vector<double>& Ws, int nfz,
-                               int nfx)
+// is -dJ/dm of the batch, divided by the illumination (the source energy
+// sum_shots Ws, or sqrt(sum Ws * sum Wr) with the adjoint wavefield's energy
+// Wr) plus 0.005 of its maximum over the window (a positive diagonal
+// preconditioner, in place of the reference's energy scaling), then shaped by
+// the [mi355x] direction options.
+static void discrete_direction(const FDWaveModel& m, psvfwi_plan* p, const std::vector<double>& Ws,
+                               const std::vector<double>* Wr, int nfz, int nfx)
 {
     const int nzt = m.grid.nzt, nxt = m.grid.nxt;
     const size_t plane = (size_t)nzt * nxt;
@@ -511,6 +556,8 @@ static void discrete_direction(const FDWaveModel& m, psvfwi_plan* p, const std::
     std::vector<double> ill(plane, 0.0);
     for (unsigned int s = 0; s < m.nshots; s++)
         for (size_t i = 0; i < plane; i++) ill[i] += Ws[s * plane + i];
+    if (Wr)
+        for (size_t i = 0; i < plane; i++) ill[i] = std::sqrt(ill[i] * (*Wr)[i]);
     double imax = 0.0;
     for (int zf = 0; zf < nfz; zf++)
         for (int xf = 0; xf < nfx; xf++) {
@@ -528,6 +575,8 @@ static void discrete_direction(const FDWaveModel& m, psvfwi_plan* p, const std::
             if (w <= 0.0) continue;
             for (int k = 0; k < 3; k++) out[k][zf][xf] = -g[k][q] / w;
         }
+    double* d[3] = {out[0][0], out[1][0], out[2][0]};
+    shape_direction(d, nfz, nfx, m.dev);
 }
 
 // CUDA_week2/fd_wave_model.cu:237-968 (the CPU branch's sequence), batched
@@ -589,7 +638,7 @@ void FDWaveModel::fwi_simulate()
         }
 
     const size_t plane = (size_t)nzt * nxt, gsz = (size_t)nfz * nfx;
-    std::vector<double> Ws(plane * nshots), Wr(plane), gl(gsz), gm(gsz), gr(gsz);
+    std::vector<double> Ws(plane * nshots), Wr(plane), WrSum(plane), gl(gsz), gm(gsz), gr(gsz);
     real_sim step = 0.01;
     iter = 0;
     L2_norm = 1000.0;
@@ -611,10 +660,21 @@ void FDWaveModel::fwi_simulate()
         misfit_history.push_back(L2_norm);
         std::cout << "L2 norm = " << L2_norm << ", max error = " << maxError << std::endl;
         for (unsigned int s = 0; s < nshots; s++) check(psvfwi_get_field(p, (int)s, 5, &Ws[s * plane]), "get_field");
+        const bool sr = discrete && dev.illumination == "source_receiver";
+        if (sr) {
+            // the discrete adjoint keeps no energy: the adjoint wavefield's, from a run
+            // without the gradient
+            check(psvfwi_adjoint(p, PSVFWI_ENERGY), "adjoint");
+            std::fill(WrSum.begin(), WrSum.end(), 0.0);
+            for (unsigned int s = 0; s < nshots; s++) {
+                check(psvfwi_get_field(p, (int)s, 5, Wr.data()), "get_field");
+                for (size_t i = 0; i < plane; i++) WrSum[i] += Wr[i];
+            }
+        }
         check(psvfwi_adjoint(p, PSVFWI_ENERGY | PSVFWI_GRADIENT), "adjoint");
 
         if (discrete) {
-            discrete_direction(*this, p, Ws, nfz, nfx);
+            discrete_direction(*this, p, Ws, sr ? &WrSum : nullptr, nfz, nfx);
         } else {
         reset_array_2d(fwi.grad_lam, nfz, nfx);
         reset_array_2d(fwi.grad_mu, nfz, nfx);

@@ -168,10 +168,24 @@ struct MI355XOptions {
     std::string gradient = "reference";  // reference (correlation) | discrete (exact misfit gradient;
                                          // implies the clean, window-correct update)
     int gradient_segment = 64;           // discrete: steps between checkpoints
+    // discrete: shaping of the descent direction on the FWI window grid (an extension; the
+    // reference's loop has none of it): illumination "source" (sum_shots Ws) or
+    // "source_receiver" (sqrt(sum Ws * sum Wr), one more adjoint run per iteration); Gaussian
+    // smoothing of sigma window cells; depth gain ((row + 1) / rows)^power; the first `mute`
+    // rows zeroed and the `taper` rows below them ramped up by a cosine
+    std::string illumination = "source";
+    double direction_smooth = 0.0;
+    double direction_depth_power = 0.0;
+    int direction_mute = 0;
+    int direction_taper = 0;
     std::string output_dir = "./io";
     bool write_csv = true;
     void parse(const char* path);
 };
+
+// Applies the direction shaping of `o` (smoothing, depth gain, mute, taper, in that order) to
+// the three [nfz][nfx] direction arrays in place.
+void shape_direction(double* d[3], int nfz, int nfx, const MI355XOptions& o);
 
 class FDWaveModel {
 public:

@@ -12,6 +12,8 @@
 //       <dir>/snap/vz_snap<k>.csv
 //   host_harness opt <L0> <L1> <L2> <s0> <s1> <s2> <out.txt>
 //       calc_opt_step of the three trial misfits and steps
+//   host_harness shape <in.bin> <nfz> <nfx> <smooth> <power> <mute> <taper> <out.bin>
+//       shape_direction on three [nfz][nfx] float64 arrays
 #include <cstdio>
 #include <filesystem>
 #include <fstream>
@@ -103,6 +105,8 @@ int run_config(char** argv)
     o << "dev " << m.dev.precision << " " << m.dev.gradient_store << " " << m.dev.checkpoint_interval << " "
       << m.dev.tile << " " << (m.dev.observed.empty() ? "-" : m.dev.observed) << " " << m.dev.gradient_scaling
       << "\n";
+    o << "direction " << m.dev.illumination << " " << m.dev.direction_smooth << " " << m.dev.direction_depth_power
+      << " " << m.dev.direction_mute << " " << m.dev.direction_taper << "\n";
     if (m.source.nseis > 0) put(o, "src_signal0", std::vector<double>(m.source.signal[0], m.source.signal[0] + m.grid.nt));
     if (m.pml.npml > 0) {
         const int tp = 2 * (m.pml.npml + 1);
@@ -124,6 +128,26 @@ int run_opt(char** argv)
     std::ofstream o(argv[8]);
     o.precision(17);
     o << calc_opt_step(L2, st) << "\n";
+    return 0;
+}
+
+int run_shape(char** argv)
+{
+    const int nfz = std::stoi(argv[3]), nfx = std::stoi(argv[4]);
+    MI355XOptions o;
+    o.direction_smooth = std::stod(argv[5]);
+    o.direction_depth_power = std::stod(argv[6]);
+    o.direction_mute = std::stoi(argv[7]);
+    o.direction_taper = std::stoi(argv[8]);
+    const size_t n = (size_t)nfz * nfx;
+    std::vector<double> a(3 * n);
+    std::ifstream in(argv[2], std::ios::binary);
+    in.read(reinterpret_cast<char*>(a.data()), (std::streamsize)(a.size() * sizeof(double)));
+    if (!in) throw std::runtime_error("shape: short input");
+    double* d[3] = {a.data(), a.data() + n, a.data() + 2 * n};
+    shape_direction(d, nfz, nfx, o);
+    std::ofstream out(argv[9], std::ios::binary);
+    out.write(reinterpret_cast<const char*>(a.data()), (std::streamsize)(a.size() * sizeof(double)));
     return 0;
 }
 
@@ -257,10 +281,11 @@ int main(int argc, char** argv)
         if (mode == "config" && argc >= 4) return run_config(argv);
         if (mode == "dropin" && argc >= 3) return run_dropin(argv[2], argc >= 4 && std::string(argv[3]) == "gpu");
         if (mode == "opt" && argc >= 9) return run_opt(argv);
+        if (mode == "shape" && argc >= 10) return run_shape(argv);
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
     }
-    std::cerr << "usage: host_harness ini|config|dropin|opt ..." << std::endl;
+    std::cerr << "usage: host_harness ini|config|dropin|opt|shape ..." << std::endl;
     return 2;
 }

@@ -489,3 +489,38 @@ def test_main_fwi_model_files(tmp_path):
     plan.close()
     # psvfwi_main prints the misfit with the stream's default 6 significant digits
     assert abs(hist[0] - m0) <= 1e-5 * m0, (hist[0], m0)
+
+
+def test_main_fwi_model_files_shaped_direction(tmp_path):
+    """The same model-file FWI with the direction options of the C++ loop (sqrt(Ws Wr)
+    illumination, which runs one more adjoint per iteration, smoothing, depth gain, mute and
+    taper): the first misfit equals the unshaped run's (the same forward), and the misfit falls
+    at every iteration."""
+    from cuda_fwi_amd import workloads as WL
+    print("psvfwi_main FWI from model files, shaped direction", flush=True)
+    nx, nz, nt, ns, nr = 160, 60, 500, 4, 24
+    w = WL.c5(nshots=ns, nrec=nr, nt=nt, nx=nx, nz=nz)
+    np.concatenate([w.vp.ravel(), w.vs.ravel(), w.rho.ravel()]).astype(np.float64).tofile(tmp_path / "true.bin")
+    vp0, vs0, rho0 = w.background
+    np.concatenate([vp0.ravel(), vs0.ravel(), rho0.ravel()]).astype(np.float64).tofile(tmp_path / "start.bin")
+    sx = [s["src"]["x"][0] for s in w.shots]
+    rx = [x for s in w.shots for x in s["rec"]["x"]]
+    rz = [z + 2 for s in w.shots for z in s["rec"]["z"]]
+    j = lambda v: ", ".join(str(a) for a in v)
+    base = MODEL_INI.format(nt=nt, nx=nx, nz=nz, nshots=ns, iters=3, sx=j(sx), sz=j([2] * ns),
+                            s0=j([0] * ns), s1=j([1] * ns), s2=j([2] * ns), sshot=j(range(ns)),
+                            samp=j([1.0] * ns), sfc=j([25.0] * ns), nrec=len(rx), rx=j(rx), rz=j(rz),
+                            r0=j([0] * len(rx)), r2=j([2] * len(rx)), rshot=j(s for s in range(ns) for _ in range(nr)))
+    hists = []
+    for extra in ("", "illumination = source_receiver\ndirection_smooth = 2\ndirection_depth_power = 2\n"
+                      "direction_mute = 4\ndirection_taper = 8\n"):
+        d = tmp_path / ("shaped" if extra else "plain")
+        d.mkdir()
+        for f in ("true.bin", "start.bin"):
+            (d / f).write_bytes((tmp_path / f).read_bytes())
+        out = _main(d, base + extra)
+        hists.append([float(l.split("=")[1].split(",")[0]) for l in out.splitlines() if l.startswith("L2 norm")])
+    print(f"  misfit per iteration: plain {hists[0]}, shaped {hists[1]}", flush=True)
+    h = hists[1]
+    assert len(h) == 3 and h[2] < h[1] < h[0]
+    assert h[0] == hists[0][0]

@@ -197,9 +197,16 @@ checkpoint_interval = 50
 tile = 16
 observed = modified
 gradient_scaling = clean
+illumination = source_receiver
+direction_smooth = 2.5
+direction_depth_power = 2
+direction_mute = 10
+direction_taper = 30
 """
     d = _config(harness, tmp_path, text)
     assert d["dev"] == ["fp64", "checkpoint", "50", "16", "modified", "clean"]
+    assert d["direction"] == ["source_receiver", "2.5", "2", "10", "30"]
+    assert _config(harness, tmp_path, REF_LIKE_INI)["direction"] == ["source", "0", "0", "0", "0"]
 
 
 def test_config_errors(harness, tmp_path):
@@ -211,6 +218,10 @@ def test_config_errors(harness, tmp_path):
     p.write_text(REF_LIKE_INI + "\n[mi355x]\nprecision = fp16\n")
     with pytest.raises(RuntimeError, match="precision"):
         run(harness, "config", p, tmp_path / "o.txt", cwd=tmp_path)
+    for bad_key in ("illumination = receiver", "direction_taper = -1"):
+        p.write_text(REF_LIKE_INI + "\n[mi355x]\n" + bad_key + "\n")
+        with pytest.raises(RuntimeError, match="illumination|direction"):
+            run(harness, "config", p, tmp_path / "o.txt", cwd=tmp_path)
 
 
 @pytest.mark.parametrize("L2,steps", [((3.0, 2.0, 2.5), (0.0, 1.0, 2.0)),
@@ -223,3 +234,30 @@ def test_calc_opt_step_dropin(harness, tmp_path, L2, steps):
     run(harness, "opt", *[repr(v) for v in L2 + steps], out)
     c = np.polyfit(steps, L2, 2)
     assert np.isclose(float(out.read_text()), -c[1] / (2 * c[0]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("smooth,power,mute,taper", [(0.0, 0.0, 0, 0), (2.5, 0.0, 0, 0), (0.0, 2.0, 3, 5),
+                                                     (1.5, 1.0, 2, 4), (3.0, 2.0, 0, 6)])
+def test_shape_direction(harness, tmp_path, smooth, power, mute, taper):
+    """shape_direction of the C++ FWI loop ([mi355x] direction_smooth, direction_depth_power,
+    direction_mute, direction_taper): scipy's gaussian_filter (mode "nearest"), then row r
+    times ((r + 1) / rows)^power, the first `mute` rows zeroed and the next `taper` rows times
+    0.5 (1 - cos(pi (r - mute + 0.5) / taper)) -- the weights of cuda_fwi_amd/fwi.py."""
+    from scipy.ndimage import gaussian_filter
+    nfz, nfx = 23, 31
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((3, nfz, nfx))
+    src, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    a.tofile(src)
+    run(harness, "shape", src, nfz, nfx, repr(smooth), repr(power), mute, taper, out)
+    got = np.fromfile(out).reshape(3, nfz, nfx)
+    r = np.arange(nfz)
+    g = ((r + 1.0) / nfz) ** power if power > 0 else np.ones(nfz)
+    g = np.where(r < mute, 0.0, g)
+    t = (r >= mute) & (r < mute + taper)
+    g[t] *= 0.5 * (1.0 - np.cos(np.pi * (r[t] - mute + 0.5) / taper))
+    for k in range(3):
+        ref = gaussian_filter(a[k], smooth, mode="nearest") if smooth > 0 else a[k]
+        assert np.allclose(got[k], ref * g[:, None], rtol=1e-12, atol=1e-14), k
+    if smooth == 0 and power == 0 and mute == 0 and taper == 0:
+        assert np.array_equal(got, a)
