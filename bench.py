@@ -748,6 +748,12 @@ def c4_extras(args, line):
                                   "workload": c2["config"]["workload"], "roofline_frac": c2["roofline"]["frac"],
                                   "tile": c2["config"]["tile"]}
         _log("1000 x 500 single shot done")
+        # the same with the time loops replayed as HIP graphs of 64 steps (its launches are
+        # short enough for the gaps between them to show)
+        a2.graph_steps = 64
+        c2g = run_c2(a2)
+        line["c2_single_shot"]["value_graph_steps_64"] = c2g["value"]
+        _log("1000 x 500 single shot with graphs done")
     if not args.no_cpu_baseline:
         w = WL.c4(nshots=1)
         window = w.model_window(dt=args.fwi_dt)
