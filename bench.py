@@ -487,7 +487,8 @@ def run_c4(args):
     form one group (checkpoints every --interval steps, recomputation not counted)."""
     from cuda_fwi_amd import psvfwi as P
     from cuda_fwi_amd import workloads as WL
-    w = WL.c4(nshots=args.nshots)
+    nx, nz = (int(v) for v in args.c4_grid.lower().split("x"))
+    w = WL.c4(nshots=args.nshots, nx=nx, nz=nz)
     if args.nt:
         for sh in w.shots:
             sh["nt"] = min(sh["nt"], args.nt)
@@ -850,6 +851,8 @@ def main():
                     help="c4: skip the FP64 group and the 1000 x 500 line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nshots", type=int, default=64)
+    ap.add_argument("--c4-grid", default="1500x600",
+                    help="c4: model cells NXxNZ (smaller grids: the same kind of model and geometry)")
     ap.add_argument("--nt", type=int, default=0, help="override nt (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only time a 1 GiB device copy (counter calibration)")

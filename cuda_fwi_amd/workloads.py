@@ -178,16 +178,19 @@ def _layered_with_fault(nx, nz, dx, seed, nlayer=8, vmin=1800.0, vmax=4200.0, of
     return vps[np.clip(layer, 0, nlayer - 1)]
 
 
-def c4(nshots=64, seed_geom=6):
-    nx, nz, dx = 1500, 600, 5.0
+def c4(nshots=64, seed_geom=6, nx=1500, nz=600):
+    """nx, nz below 1500 x 600 give the same kind of model and geometry on a smaller grid
+    (source positions and receiver counts scaled with nx)."""
+    dx = 5.0
     vp = _layered_with_fault(nx, nz, dx, 5)
     vs = vp / 1.73
     rng = np.random.default_rng(seed_geom)
+    sc = nx / 1500.0
     shots = []
     for _ in range(nshots):
         nt = int(rng.integers(3000, 8001))
-        nrec = int(rng.integers(50, 401))
-        xs = int(rng.integers(100, 1401))
+        nrec = max(2, int(round(int(rng.integers(50, 401)) * min(1.0, sc))))
+        xs = int(round(int(rng.integers(100, 1401)) * sc))
         shots.append(dict(nt=nt, src=dict(x=[xs], z=[2], comp=[2]), rec=_line(nx, nrec), amp=1.0))
     ntmax = max(s["nt"] for s in shots)
     w = Workload("c4", nx, nz, dx, 5e-4, ntmax, 40, 4, 0, 12.0, vp, vs, _gardner(vp), shots)
