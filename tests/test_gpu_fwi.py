@@ -76,3 +76,22 @@ def test_receiver_energy_run_leaves_the_discrete_gradient_unchanged():
     assert (na, nb) == (2, 4)
     z1, z2, x1, x2 = win
     assert np.all(wsr[z1:z2, x1:x2] > 0) and not np.allclose(wsr, ws)
+
+
+def test_fwi_direction_options_reduce_misfit_and_model_error():
+    """The direction options together (sqrt(Ws Wr) illumination, smoothing, depth gain, mute
+    and taper) on the same small model: the misfit falls at every iteration, the model error
+    falls, and the muted rows keep the start model."""
+    print("host FWI loop with the direction options, 240 x 80, 6 shots, 3 iterations", flush=True)
+    w, plan, shots, observed, truth, win = _setup()
+    start = w.padded(*w.background)
+    inv = Inversion(plan, shots, observed, start, win, segment=50, precond="source_receiver", smooth=2.0,
+                    depth_power=2.0, mute=3, taper=6)
+    e0 = inv.model_error(truth)
+    h = inv.run(3, truth=truth, log=print)
+    print(f"  misfit {h.misfit}; vp error {e0:.5f} -> {h.model_error}")
+    assert all(b < a for a, b in zip(h.misfit, h.misfit[1:]))
+    assert h.model_error[-1] < e0
+    z1, z2, x1, x2 = win
+    for k, i in (("lam", 0), ("mu", 1), ("rho", 2)):
+        assert np.array_equal(inv.m[k][z1:z1 + 3, x1:x2], start[i][z1:z1 + 3, x1:x2])
