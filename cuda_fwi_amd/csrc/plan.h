@@ -249,6 +249,7 @@ private:
     void drop_results();
     psvfwi_status alloc_store(int ntf);
     size_t ck_bytes_per_slot() const;
+    // k = 0: one entry per step plus the final state (every state kept)
     psvfwi_status alloc_checkpoints(int k);
 
     // ---------------------------------------------------------- plan_shots.cpp
@@ -275,7 +276,7 @@ private:
 
     // ------------------------------------------------------- plan_discrete.cpp
     psvfwi_status adjoint_discrete();
-    View entry(int e) const;
+    View entry(const T* base, int e) const;
     psv::DiscreteParams<T> dparams(int it, const View& pre, const View& post);
 
     // -------------------------------------------------------- plan_results.cpp

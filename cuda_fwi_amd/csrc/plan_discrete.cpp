@@ -6,7 +6,8 @@ namespace psvplan {
 
 // Discrete gradient: per segment (last first), restore its checkpoint,
 // replay it keeping the state before every step and after the last, then
-// run the transposed steps back through it.
+// run the transposed steps back through it.  With segments of one step the
+// forward kept every state, and the transposed steps read those directly.
 template <typename T>
 psvfwi_status Plan<T>::adjoint_discrete()
 {
@@ -31,12 +32,16 @@ psvfwi_status Plan<T>::adjoint_discrete()
         PSV_HIP(dg_[i].alloc(nf));
         PSV_HIP(psv::launch_fill<T>(dg_[i].p, nf, (T)0, stream_));
     }
-    PSV_TRY(alloc_state(replay_));
-    if (fused()) {
-        PSV_TRY(alloc_state(replay2_));
-        PSV_TRY(zero_state(replay2_));
+    // segments of one step: the forward kept every state, no replay
+    const bool every = kk == 1;
+    if (!every) {
+        PSV_TRY(alloc_state(replay_));
+        if (fused()) {
+            PSV_TRY(alloc_state(replay2_));
+            PSV_TRY(zero_state(replay2_));
+        }
+        PSV_HIP(seg_.reserve((size_t)(std::min(kk, ntmax_) + 1) * state_elems()));
     }
-    PSV_HIP(seg_.reserve((size_t)(std::min(kk, ntmax_) + 1) * state_elems()));
     // receiver groups of each slot, for the adjoint sources
     const std::vector<int>& go = adj_tab_.grp_off;
     int maxg = 0;
@@ -79,20 +84,24 @@ psvfwi_status Plan<T>::adjoint_discrete()
         const int nseg = (ntg + kk - 1) / kk;
         for (int c = nseg - 1; c >= 0; c--) {
             const int t0 = c * kk, t1 = std::min(t0 + kk, ntg);
-            PSV_TRY(restore_checkpoint(c, active(t0)));
-            PSV_TRY(state_copy<true>(seg_.p, active(t0), replay_));
-            for (int it = t0; it < t1; it++) {
-                psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
-                p.it = it;
-                p.nslots = active(it);
-                p.flags = 0;
-                p.We = nullptr;
-                PSV_TRY(launch_plain(p, psv::kForward, replay_, replay2_));
-                PSV_TRY(state_copy<true>(seg_.p + (size_t)(it - t0 + 1) * state_elems(), active(it), replay_));
-                launches_++;
+            // the states before steps t0 .. t1 - 1 and after t1 - 1
+            const T* base = every ? ckpt_.p + (size_t)t0 * state_elems() : seg_.p;
+            if (!every) {
+                PSV_TRY(restore_checkpoint(c, active(t0)));
+                PSV_TRY(state_copy<true>(seg_.p, active(t0), replay_));
+                for (int it = t0; it < t1; it++) {
+                    psv::StepParams<T> p = base_params(replay_, fwd_tab_noreceivers_, true);
+                    p.it = it;
+                    p.nslots = active(it);
+                    p.flags = 0;
+                    p.We = nullptr;
+                    PSV_TRY(launch_plain(p, psv::kForward, replay_, replay2_));
+                    PSV_TRY(state_copy<true>(seg_.p + (size_t)(it - t0 + 1) * state_elems(), active(it), replay_));
+                    launches_++;
+                }
             }
             for (int it = t1 - 1; it >= t0; it--) {
-                psv::DiscreteParams<T> d = dparams(it, entry(it - t0), entry(it - t0 + 1));
+                psv::DiscreteParams<T> d = dparams(it, entry(base, it - t0), entry(base, it - t0 + 1));
                 d.max_groups = maxg;
                 // receivers: the stress pass of step it + 1 added the sources
                 // of the shots active there; the rest start at step it
@@ -120,9 +129,9 @@ psvfwi_status Plan<T>::adjoint_discrete()
 }
 
 template <typename T>
-typename Plan<T>::View Plan<T>::entry(int e) const
+typename Plan<T>::View Plan<T>::entry(const T* states, int e) const
 {
-    const T* base = seg_.p + (size_t)e * state_elems();
+    const T* base = states + (size_t)e * state_elems();
     View v{};
     size_t off = 0;
     const size_t w0 = (size_t)w0_;
@@ -263,7 +272,7 @@ psvfwi_status Plan<T>::get_model_gradient(int shot, double* gl, double* gm, doub
 
 #define PSV_INSTANTIATE(T) \
     template psvfwi_status Plan<T>::adjoint_discrete(); \
-    template Plan<T>::View Plan<T>::entry(int) const; \
+    template Plan<T>::View Plan<T>::entry(const T*, int) const; \
     template psv::DiscreteParams<T> Plan<T>::dparams(int, const View&, const View&); \
     template psvfwi_status Plan<T>::get_coefficient_gradient(int, double*, double*, double*, double*, double*) const; \
     template psvfwi_status Plan<T>::get_model_gradient(int, double*, double*, double*) const;

@@ -26,11 +26,15 @@ psvfwi_status Plan<T>::forward(int flags)
     const bool rb = grad && !discrete && store_mode_ == PSVFWI_STORE_RANDOM_BOUNDARY;
     if (rb) PSV_TRY(rb_check());
     const int kint = discrete ? disc_k_ : ck_k_;
+    // discrete gradient with segments of one step: every state is kept (entry
+    // t = the state before step t, entry nt = the final state) and the adjoint
+    // reads them without a replay
+    const bool every = discrete && kint == 1;
     // the store is not cleared: the adjoint reads exactly the entries the
     // forward writes, and get_store zeroes the ones the window loops never
     // write
     if (full) PSV_TRY(alloc_store(1 + (ntmax_ - 1) / w_.dt));
-    if (ckpt) PSV_TRY(alloc_checkpoints(kint));
+    if (ckpt) PSV_TRY(alloc_checkpoints(every ? 0 : kint));
     const bool snap = snap_interval_ > 0;
     if (snap) PSV_HIP(snaps_.alloc((size_t)((ntmax_ - 1) / snap_interval_ + 1) * nslots_ * fst_));
     snaps_taken_ = snap;
@@ -68,13 +72,16 @@ psvfwi_status Plan<T>::forward(int flags)
                     if (it >= ntg) break;
                 }
             }
-            if (ckpt && it % kint == 0) PSV_TRY(save_checkpoint(it / kint, na));
+            if (ckpt && (every ? it == 0 : it % kint == 0)) PSV_TRY(save_checkpoint(it / kint, na));
             psv::StepParams<T> p = base_params(main_, fwd_tab_, true);
             p.it = it;
             p.nslots = active(it);
             p.flags = fflags;
             p.tf0 = 0;
             PSV_TRY(launch_timed(p, psv::kForward, 0, main_, main2_));
+            // after the step, for the shots it advanced: a shot's final state is
+            // entry nt_s, which the next step's shots (a prefix) may not include
+            if (every) PSV_TRY(save_checkpoint(it + 1, na));
             if (snap && it % snap_interval_ == 0) {
                 // forward_kernel_PSV_GPU.cu:1067-1080: after the step, every snap_interval
                 const size_t k = (size_t)(it / snap_interval_);

@@ -277,7 +277,7 @@ size_t Plan<T>::ck_bytes_per_slot() const
 template <typename T>
 psvfwi_status Plan<T>::alloc_checkpoints(int k)
 {
-    ncp_ = (ntmax_ + k - 1) / k;
+    ncp_ = k > 0 ? (ntmax_ + k - 1) / k : ntmax_ + 1;
     PSV_HIP(ckpt_.alloc((size_t)ncp_ * state_elems()));
     return PSVFWI_OK;
 }

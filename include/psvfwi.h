@@ -246,7 +246,8 @@ psvfwi_status psvfwi_get_gradient(const psvfwi_plan* plan, int shot, double* gra
  * is used: the gradients cover the whole grid.  Sources must be vx or vz
  * forces (comp 1 or 2), and the CPML profile must have a = a_half = 0 at index
  * npml (psvfwi_cpml_profiles does).  Device memory: (segment + 1) full states
- * per shot besides the checkpoints. */
+ * per shot besides the checkpoints.  segment == 1: the forward keeps every
+ * state (nt + 1 full states per shot) and the adjoint runs no replay. */
 #define PSVFWI_GRADIENT_REFERENCE 0 /* correlation gradient of the reference (default) */
 #define PSVFWI_GRADIENT_DISCRETE 1  /* exact gradient of the discrete misfit */
 psvfwi_status psvfwi_set_gradient_mode(psvfwi_plan* plan, int mode, int segment);

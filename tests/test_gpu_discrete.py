@@ -39,7 +39,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"o{c['fdorder']}fs{c['fsurf']}p{c['npml']}")
-@pytest.mark.parametrize("segment,tz", [(7, 8), (64, -16), (500, 16)])
+@pytest.mark.parametrize("segment,tz", [(7, 8), (64, -16), (500, 16), (1, 8), (1, -16)])
 def test_discrete_gradient_fp64_bit_exact(case, segment, tz):
     print(f"discrete fp64 {case} segment {segment} tile {tz}", flush=True)
     pb = RC.small_problem(nx=50, nz=36, nt=160, seed=4, nrec=6, **case)
@@ -80,10 +80,13 @@ def test_discrete_gradient_fp32_bit_exact_and_tolerance(fdorder):
     assert max(err.values()) < 1e-4
 
 
-def test_discrete_gradient_batched_uneven_shots():
+@pytest.mark.parametrize("segment", [33, 1])
+def test_discrete_gradient_batched_uneven_shots(segment):
     """Shots of different nt and receiver counts in one batch: each shot's
-    gradient is its own oracle gradient; the batch sum is their FP64 sum."""
-    print("discrete batched", flush=True)
+    gradient is its own oracle gradient; the batch sum is their FP64 sum.
+    Segment 1: the forward keeps every state (a shot's final state while the
+    longer shots go on) and the adjoint reads them without a replay."""
+    print(f"discrete batched, segment {segment}", flush=True)
     base = RC.small_problem(nx=56, nz=40, npml=8, nt=10, fdorder=4, fsurf=1, seed=2)
     rng = np.random.default_rng(12)
     shots, obs, refs = [], [], []
@@ -103,7 +106,7 @@ def test_discrete_gradient_batched_uneven_shots():
         pb.rec = dict(rec, shot=[0] * nrec)
         _, res, _ = O.l2_adjoint_sources(pb.dt, O.forward(pb)["rec"].copy(), o)
         refs.append(O.discrete_gradient(pb, res))
-    plan, _ = _device_gradient(base, obs, "fp64", 33, shots=shots)
+    plan, _ = _device_gradient(base, obs, "fp64", segment, shots=shots)
     for i in range(len(shots)):
         g = plan.coefficient_gradient(i)
         for k in COEF:

@@ -32,12 +32,12 @@ def _shots(base, comps=(2, 4, 2, 1, 2), seed=9):
 
 
 def _run(base, shots, obs, group, store="full", interval=0, gradient="reference", graph=0, snaps=0,
-         prec="fp32", tz=8):
+         prec="fp32", tz=8, segment=40):
     plan = plan_of(base, prec, shots=shots, tz=tz)
     plan.set_window(RC.full_window(base, dt=2))
     plan.set_store_mode(store, interval)
     if gradient == "discrete":
-        plan.set_gradient_mode("discrete", 40)
+        plan.set_gradient_mode("discrete", segment)
     plan.set_graph_steps(graph)
     if snaps:
         plan.set_snapshots(snaps, "vz")
@@ -95,6 +95,8 @@ def test_launch_groups_discrete_gradient():
     ref = _run(base, shots, obs, 0, gradient="discrete", prec="fp64")
     assert np.abs(ref[-1]).max() > 0
     _same(_run(base, shots, obs, 2, gradient="discrete", prec="fp64"), ref)
+    # segments of one step: every state kept by the forward, no replay
+    _same(_run(base, shots, obs, 3, gradient="discrete", prec="fp64", segment=1), ref)
 
 
 def test_launch_groups_random_boundary():
