@@ -307,7 +307,8 @@ class Plan:
     def set_gradient_mode(self, mode, segment=64):
         """'reference' (the correlation gradient of adjoint_kernel_PSV) or
         'discrete' (the exact gradient of the misfit of this scheme, with
-        checkpoints every `segment` steps)."""
+        checkpoints every `segment` steps; segment 1 keeps every state of the
+        forward and the adjoint replays nothing)."""
         m = {"reference": 0, "discrete": 1}[mode] if isinstance(mode, str) else int(mode)
         _check(lib().psvfwi_set_gradient_mode(self._h, m, int(segment)))
 

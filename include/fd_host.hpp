@@ -167,7 +167,7 @@ struct MI355XOptions {
     std::string gradient_scaling = "reference";  // reference | clean (energy weights, model update)
     std::string gradient = "reference";  // reference (correlation) | discrete (exact misfit gradient;
                                          // implies the clean, window-correct update)
-    int gradient_segment = 64;           // discrete: steps between checkpoints
+    int gradient_segment = 64;           // discrete: steps between checkpoints (1: every state kept)
     // discrete: shaping of the descent direction on the FWI window grid (an extension; the
     // reference's loop has none of it): illumination "source" (sum_shots Ws) or
     // "source_receiver" (sqrt(sum Ws * sum Wr), one more adjoint run per iteration); Gaussian
