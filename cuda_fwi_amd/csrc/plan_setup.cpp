@@ -278,7 +278,14 @@ template <typename T>
 psvfwi_status Plan<T>::alloc_checkpoints(int k)
 {
     ncp_ = k > 0 ? (ntmax_ + k - 1) / k : ntmax_ + 1;
-    PSV_HIP(ckpt_.alloc((size_t)ncp_ * state_elems()));
+    const hipError_t e = ckpt_.alloc((size_t)ncp_ * state_elems());
+    if (e == hipErrorOutOfMemory && k == 0) {
+        (void)hipGetLastError();
+        return fail(PSVFWI_E_NOMEM, "forward: keeping every state (discrete gradient, segment 1) needs " +
+                                        std::to_string((double)ncp_ * state_elems() * sizeof(T) / 1e9) +
+                                        " GB of device memory; use a longer segment");
+    }
+    PSV_HIP(e);
     return PSVFWI_OK;
 }
 
