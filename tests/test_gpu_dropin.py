@@ -367,6 +367,13 @@ def test_main_fwi_discrete_gradient_converges(tmp_path):
     e1 = np.linalg.norm(lam - pb_true.lam)
     print(f"  lambda error vs the disc model: start {e0:.4e}, after {iters} iterations {e1:.4e}")
     assert e1 < e0
+    # gradient_segment = 1: the forward keeps every state and the adjoint
+    # replays nothing; the same gradients, so the same run to the printed digit
+    every = tmp_path / "every"
+    every.mkdir()
+    out1 = _main(every, text.replace("gradient_segment = 50", "gradient_segment = 1"))
+    assert _printed_l2(out1) == l2
+    assert np.array_equal(_csv(every / f"io/density_snap/lam_snap{iters}.csv"), lam)
 
 
 MODEL_INI = """\
